@@ -1,0 +1,160 @@
+/*
+ * mosscap_hip.h - C ABI of the MI355X (gfx950) finite-volume Euler hot path.
+ *
+ * Drop-in for the operator slots Mosscap's driver calls (Simulation.hpp:38-62):
+ * compute_dt (Hydro.hpp:16, Hydro.cpp:208-242), time_step
+ * (TimeStepping.cpp:39-308) and fill_bcs (Boundaries.hpp:132-149). Arrays
+ * use the reference's layout: Q[var][k][j][i], x fastest, ghost cells
+ * included, var = rho, m_x[, m_y[, m_z]], E (State.hpp:10-26).
+ *
+ * Conventions: every function returns MH_OK (0) or an MH_ERR_* code and never
+ * throws; mh_last_error() gives the message. Calls are synchronous on return,
+ * like the reference's fenced kernels (LoopUtils.hpp:253-255), except
+ * mh_run, which returns once its steps have completed. A context is bound to
+ * the HIP device current at mh_create and owns its device buffers and stream;
+ * it must not be used from two threads at once.
+ */
+#ifndef MOSSCAP_HIP_H
+#define MOSSCAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK 0
+#define MH_ERR_INVALID 1     /* bad argument; the reference throws std::runtime_error (SetupModules.hpp:146) */
+#define MH_ERR_UNSUPPORTED 2 /* valid for the reference but outside this library (e.g. 3-D, tracers) */
+#define MH_ERR_HIP 3         /* HIP runtime or launch failure */
+
+#define MH_FORM_FP64 0  /* drop-in: FP64 state and arithmetic, bitwise the reference CPU build */
+#define MH_FORM_FP32C 1 /* precision form: FP32 reconstruction/Riemann, compensated hi+lo FP32 state,
+                           FP64 pressure recovery and CFL reduction */
+
+typedef struct mh_ctx mh_ctx;
+
+/* GridSize, State.hpp:101-106: counts include 2*ng ghosts; unused axes are 1. */
+typedef struct {
+    int32_t xc, yc, zc, ng;
+} mh_grid_size;
+
+/* GridLoc, State.hpp:108-112. */
+typedef struct {
+    double x, y, z;
+} mh_grid_loc;
+
+/* Boundaries, State.hpp:82-98; values are BoundaryType (State.hpp:62-70):
+ * 0 wall, 1 periodic, 2 symmetric, 3 symmetricoutflowdiode, 4 zerograd,
+ * 5 constant, 6 userfn (ghosts left untouched for the caller's hook). */
+typedef struct {
+    int32_t xs, xe, ys, ye, zs, ze;
+    double xs_const[5], xe_const[5], ys_const[5], ye_const[5], zs_const[5], ze_const[5];
+} mh_boundaries;
+
+/* NumericalSchemes, Simulation.hpp:17-22: reconstruction 0 fog, 1 muscl,
+ * 2 ppm, 3 weno5z; slope_limit 0 vanleer, 1 monotonizedcentral, 2 minmod;
+ * riemann_solver 0 rusanov, 1 hll, 2 hllc; time_stepper 0 rk2, 1 ssprk3,
+ * 2 ssprk4 (the 4-stage third-order SSPRK(4,3)). */
+typedef struct {
+    int32_t reconstruction, slope_limit, riemann_solver, time_stepper;
+} mh_schemes;
+
+/* Library options (no counterpart in the reference). Zero-initialised
+ * options select FP64 and the tuned defaults. */
+typedef struct {
+    int32_t precision_form; /* MH_FORM_* */
+    int32_t span_rows;      /* rows marched per wavefront in 2-D; 0 = tuned default */
+    float rho_floor;        /* FP32C positivity floors on the reconstruction input; <= 0 disables */
+    float p_floor;
+} mh_options;
+
+typedef struct {
+    uint64_t floor_hits; /* FP32C: (interior cell, stage) pairs whose rho or p was floored */
+    int64_t steps;       /* steps taken by this context */
+    double last_dt;      /* dt of the last step */
+    double time;         /* device clock after mh_run */
+    double prev_output_time;
+} mh_stats;
+
+/* Build a context (replaces setup_grid's allocations, SetupModules.hpp:9-64,
+ * plus select_hydro_fns / select_timestepper, Hydro.cpp:244-323 and
+ * TimeStepping.cpp:331-344). num_dim 1 or 2; num_tracers must be 0. */
+int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const mh_boundaries* bdry,
+              int32_t num_dim, int32_t num_tracers, const mh_schemes* schemes, double gamma,
+              const mh_options* opts, mh_ctx** out);
+void mh_destroy(mh_ctx* ctx);
+/* Message of the last failure on ctx, or of the last failed mh_create on this thread when ctx is NULL. */
+const char* mh_last_error(const mh_ctx* ctx);
+
+/* Number of doubles in Q: nvar * zc * yc * xc. */
+int64_t mh_num_values(const mh_ctx* ctx);
+
+/* Move the conserved state Q (ghosts included) between host and device. */
+int mh_upload(mh_ctx* ctx, const double* q_host);
+int mh_download(mh_ctx* ctx, double* q_host);
+/* Same, from/to device memory on the context's device (no PCIe transfer). */
+int mh_upload_device(mh_ctx* ctx, const double* q_dev);
+int mh_download_device(mh_ctx* ctx, double* q_dev);
+
+/* fill_bcs(sim), Boundaries.hpp:132-149 (user_bc is the caller's). */
+int mh_fill_bcs(mh_ctx* ctx);
+
+/* compute_dt(sim), Hydro.cpp:208-242: CFL reduction over all cells incl.
+ * ghosts, times max_cfl, clamped to min(max_time, prev_output_time + delta). */
+int mh_compute_dt(mh_ctx* ctx, double max_cfl, double time, double max_time, double prev_output_time,
+                  double delta, double* dt_out);
+
+/* sim.time_step(sim, dt), TimeStepping.cpp:39-308: every stage, ghost fill
+ * after each, ending with ghosts filled. */
+int mh_time_step(mh_ctx* ctx, double dt);
+
+/* One stage of the scheme (stage 0 snapshots Q^n; later stages use it);
+ * ghosts of the result are filled. mh_time_step == stages 0..n-1. */
+int mh_stage(mh_ctx* ctx, int32_t stage, double dt);
+
+/* main.cpp:54-71 step loop on the device: each step computes dt (fused into
+ * the previous step's last stage), advances and does the output-time
+ * bookkeeping, for nsteps steps (steps past max_time have dt = 0). *time and
+ * *prev_output_time are read and updated. With use_graph, steps replay a
+ * captured HIP graph. */
+int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max_time, double* prev_output_time,
+           double delta, int32_t use_graph);
+
+/* Per-kernel timing of a device step loop (no counterpart in the reference,
+ * which only has YAKL auto-profile timers, LoopUtils.hpp:200-202): as mh_run
+ * without graphs, with HIP events recorded around every launch on the
+ * context's stream. Totals in milliseconds. */
+typedef struct {
+    double stage_ms;        /* fused stage kernels */
+    int64_t stage_launches;
+    double ghost_ms;        /* ghost-fill kernels */
+    int64_t ghost_launches;
+    double clock_ms;        /* dt finalize kernels */
+    int64_t clock_launches;
+    double step_ms;         /* whole loop, first event to last */
+} mh_timing;
+
+int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max_time,
+                 double* prev_output_time, double delta, mh_timing* out);
+
+int mh_get_stats(mh_ctx* ctx, mh_stats* out);
+
+/* HIP stream the context launches on (a hipStream_t), for event timing. */
+void* mh_stream(mh_ctx* ctx);
+
+/* ---- functor probes (run the device functors on n inputs; host arrays) ---- */
+/* reconstruct<recon, limiter>: stencils[n][5] (s[-2..2]) -> wl[n], wr[n]; prec 0 FP64, 1 FP32. */
+int mh_probe_reconstruct(int32_t recon, int32_t limiter, int32_t prec, int64_t n, const double* stencils,
+                         double* wl, double* wr);
+/* riemann_flux<solver, axis, nd>: wl, wr, flux are [n][2+nd] primitive / flux vectors. */
+int mh_probe_riemann(int32_t solver, int32_t axis, int32_t nd, int32_t prec, double gamma, int64_t n,
+                     const double* wl, const double* wr, double* flux);
+/* op 0: a/b, 1: sqrt(a) in FP64 as the drop-in computes them; 2, 3: the FP32 forms. */
+int mh_probe_math(int32_t op, int64_t n, const double* a, const double* b, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOSSCAP_HIP_H */

@@ -1,0 +1,329 @@
+"""MI355X-native finite-volume Euler hot path of Mosscap: Python bindings.
+
+The product is the HIP library ``libmosscap_hip.so`` behind the C ABI in
+``include/mosscap_hip.h``; this module binds it with ctypes for the test
+harness, the benchmark and the adjoint's ``torch.autograd.Function``. There is
+no CPU fallback: every entry point fails loudly when the library (or a GPU) is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libmosscap_hip.so"
+
+# Enum values, as in the reference's name tables (Reconstruct.hpp:14-31,
+# Riemann.hpp:15-19, TimeStepping.hpp:15-19, State.hpp:71-79).
+RECONSTRUCTION = {"fog": 0, "muscl": 1, "ppm": 2, "weno5z": 3}
+SLOPE_LIMITER = {"vanleer": 0, "monotonizedcentral": 1, "minmod": 2}
+RIEMANN_SOLVER = {"rusanov": 0, "hll": 1, "hllc": 2}
+TIME_STEPPER = {"rk2": 0, "ssprk3": 1, "ssprk4": 2}
+BOUNDARY = {"wall": 0, "periodic": 1, "symmetric": 2, "symmetricoutflowdiode": 3, "zerograd": 4,
+            "constant": 5, "userfn": 6}
+MIN_GHOST = {"fog": 1, "muscl": 2, "ppm": 3, "weno5z": 3}  # Reconstruct.hpp:47-61
+NUM_STAGES = {"rk2": 2, "ssprk3": 3, "ssprk4": 4}
+FP64, FP32C = 0, 1
+
+MH_OK, MH_ERR_INVALID, MH_ERR_UNSUPPORTED, MH_ERR_HIP = 0, 1, 2, 3
+
+
+class MhGridSize(C.Structure):
+    _fields_ = [("xc", C.c_int32), ("yc", C.c_int32), ("zc", C.c_int32), ("ng", C.c_int32)]
+
+
+class MhGridLoc(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class MhBoundaries(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("xs", "xe", "ys", "ye", "zs", "ze")] + \
+               [(n + "_const", C.c_double * 5) for n in ("xs", "xe", "ys", "ye", "zs", "ze")]
+
+
+class MhSchemes(C.Structure):
+    _fields_ = [("reconstruction", C.c_int32), ("slope_limit", C.c_int32), ("riemann_solver", C.c_int32),
+                ("time_stepper", C.c_int32)]
+
+
+class MhOptions(C.Structure):
+    _fields_ = [("precision_form", C.c_int32), ("span_rows", C.c_int32), ("rho_floor", C.c_float),
+                ("p_floor", C.c_float)]
+
+
+class MhStats(C.Structure):
+    _fields_ = [("floor_hits", C.c_uint64), ("steps", C.c_int64), ("last_dt", C.c_double), ("time", C.c_double),
+                ("prev_output_time", C.c_double)]
+
+
+class MhTiming(C.Structure):
+    _fields_ = [("stage_ms", C.c_double), ("stage_launches", C.c_int64), ("ghost_ms", C.c_double),
+                ("ghost_launches", C.c_int64), ("clock_ms", C.c_double), ("clock_launches", C.c_int64),
+                ("step_ms", C.c_double)]
+
+
+class MoscapError(RuntimeError):
+    """A non-zero status from the C ABI (the reference throws std::runtime_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+# (name, restype, argtypes) of every symbol include/mosscap_hip.h declares
+_SIGNATURES = [
+    ("mh_create", C.c_int, [C.POINTER(MhGridSize), C.c_double, C.POINTER(MhGridLoc), C.POINTER(MhBoundaries),
+                            C.c_int32, C.c_int32, C.POINTER(MhSchemes), C.c_double, C.POINTER(MhOptions),
+                            C.POINTER(C.c_void_p)]),
+    ("mh_destroy", None, [C.c_void_p]),
+    ("mh_last_error", C.c_char_p, [C.c_void_p]),
+    ("mh_num_values", C.c_int64, [C.c_void_p]),
+    ("mh_upload", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_download", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_upload_device", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_download_device", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_fill_bcs", C.c_int, [C.c_void_p]),
+    ("mh_compute_dt", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.POINTER(C.c_double)]),
+    ("mh_time_step", C.c_int, [C.c_void_p, C.c_double]),
+    ("mh_stage", C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    ("mh_run", C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_double), C.c_double,
+                         C.POINTER(C.c_double), C.c_double, C.c_int32]),
+    ("mh_run_timed", C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_double), C.c_double,
+                               C.POINTER(C.c_double), C.c_double, C.POINTER(MhTiming)]),
+    ("mh_get_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
+    ("mh_stream", C.c_void_p, [C.c_void_p]),
+    ("mh_probe_reconstruct", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+    ("mh_probe_riemann", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int64, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
+    ("mh_probe_math", C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+]
+SYMBOLS = [s[0] for s in _SIGNATURES]
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (built in-tree by ``python -m mosscap_amd.build``)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m mosscap_amd.build` "
+                              "(there is no CPU fallback)")
+        h = C.CDLL(str(LIB_PATH))
+        for name, res, args in _SIGNATURES:
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _check(code: int, ctx=None) -> None:
+    if code != MH_OK:
+        msg = lib().mh_last_error(ctx)
+        raise MoscapError(code, msg.decode() if msg else "unknown error")
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class Config:
+    """Problem/scheme configuration (the YAML keys of SetupModules.hpp:9-177)."""
+    num_dim: int = 2
+    nx: int = 256
+    ny: int = 256
+    ng: int = 3
+    x_dim: float = 1.0
+    x_start: float = 0.0
+    y_start: float = 0.0
+    bc: tuple = ("wall", "wall", "wall", "wall")  # xs, xe, ys, ye
+    bc_const: tuple = ((0.0,) * 5,) * 4
+    reconstruction: str = "muscl"
+    slope_limiter: str = "monotonizedcentral"
+    riemann_solver: str = "hll"
+    time_stepper: str = "ssprk3"
+    gamma: float = 1.4
+    max_cfl: float = 0.8
+    max_time: float = 1.0
+    output_delta: float = 0.1
+
+    @property
+    def xc(self) -> int:
+        return self.nx + 2 * self.ng
+
+    @property
+    def yc(self) -> int:
+        return self.ny + 2 * self.ng if self.num_dim > 1 else 1
+
+    @property
+    def nvar(self) -> int:
+        return 2 + self.num_dim
+
+    @property
+    def dx(self) -> float:
+        return self.x_dim / self.nx  # setup_grid, SetupModules.hpp:29-31
+
+    @property
+    def shape(self) -> tuple:
+        return (self.nvar, 1, self.yc, self.xc)
+
+
+class HydroContext:
+    """One problem on the GPU: owns the C-ABI context."""
+
+    def __init__(self, cfg: Config, precision_form: int = FP64, span_rows: int = 0,
+                 rho_floor: float = 0.0, p_floor: float = 0.0):
+        L = lib()
+        self.cfg = cfg
+        sz = MhGridSize(cfg.xc, cfg.yc, 1, cfg.ng)
+        loc = MhGridLoc(cfg.x_start, cfg.y_start, 0.0)
+        b = MhBoundaries()
+        names = ("xs", "xe", "ys", "ye")
+        for n, kind, cv in zip(names, cfg.bc, cfg.bc_const):
+            setattr(b, n, BOUNDARY[kind.lower()])
+            arr = getattr(b, n + "_const")
+            for i, v in enumerate(cv):
+                arr[i] = v
+        sch = MhSchemes(RECONSTRUCTION[cfg.reconstruction], SLOPE_LIMITER[cfg.slope_limiter],
+                        RIEMANN_SOLVER[cfg.riemann_solver], TIME_STEPPER[cfg.time_stepper])
+        opt = MhOptions(precision_form, span_rows, rho_floor, p_floor)
+        h = C.c_void_p()
+        _check(L.mh_create(C.byref(sz), cfg.dx, C.byref(loc), C.byref(b), cfg.num_dim, 0, C.byref(sch), cfg.gamma,
+                           C.byref(opt), C.byref(h)))
+        self._h = h
+        self.n_values = L.mh_num_values(h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mh_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _c(self, code):
+        _check(code, self._h)
+
+    def upload(self, q: np.ndarray) -> None:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == self.n_values, (q.shape, self.n_values)
+        self._c(lib().mh_upload(self._h, _ptr(q)))
+
+    def download(self) -> np.ndarray:
+        q = np.empty(self.cfg.shape, dtype=np.float64)
+        self._c(lib().mh_download(self._h, _ptr(q)))
+        return q
+
+    def upload_device(self, ptr: int) -> None:
+        self._c(lib().mh_upload_device(self._h, C.c_void_p(ptr)))
+
+    def download_device(self, ptr: int) -> None:
+        self._c(lib().mh_download_device(self._h, C.c_void_p(ptr)))
+
+    def fill_bcs(self) -> None:
+        self._c(lib().mh_fill_bcs(self._h))
+
+    def compute_dt(self, time: float, prev_output_time: float, max_cfl: float | None = None,
+                   max_time: float | None = None, delta: float | None = None) -> float:
+        c = self.cfg
+        dt = C.c_double()
+        self._c(lib().mh_compute_dt(self._h, c.max_cfl if max_cfl is None else max_cfl, time,
+                                    c.max_time if max_time is None else max_time, prev_output_time,
+                                    c.output_delta if delta is None else delta, C.byref(dt)))
+        return dt.value
+
+    def time_step(self, dt: float) -> None:
+        self._c(lib().mh_time_step(self._h, dt))
+
+    def stage(self, stage: int, dt: float) -> None:
+        self._c(lib().mh_stage(self._h, stage, dt))
+
+    def run(self, nsteps: int, time: float = 0.0, prev_output_time: float = 0.0, use_graph: bool = True,
+            max_cfl: float | None = None, max_time: float | None = None, delta: float | None = None):
+        c = self.cfg
+        t = C.c_double(time)
+        p = C.c_double(prev_output_time)
+        self._c(lib().mh_run(self._h, nsteps, c.max_cfl if max_cfl is None else max_cfl, C.byref(t),
+                             c.max_time if max_time is None else max_time, C.byref(p),
+                             c.output_delta if delta is None else delta, 1 if use_graph else 0))
+        return t.value, p.value
+
+    def run_timed(self, nsteps: int, time: float = 0.0, prev_output_time: float = 0.0) -> tuple:
+        c = self.cfg
+        t = C.c_double(time)
+        p = C.c_double(prev_output_time)
+        tm = MhTiming()
+        self._c(lib().mh_run_timed(self._h, nsteps, c.max_cfl, C.byref(t), c.max_time, C.byref(p),
+                                   c.output_delta, C.byref(tm)))
+        return t.value, p.value, tm
+
+    def stats(self) -> MhStats:
+        s = MhStats()
+        self._c(lib().mh_get_stats(self._h, C.byref(s)))
+        return s
+
+    @property
+    def stream(self) -> int:
+        return lib().mh_stream(self._h) or 0
+
+
+def run_to_time(ctx: HydroContext, time: float = 0.0, prev_output_time: float = 0.0, max_steps: int = -1):
+    """main.cpp:54-71 on the host: while time < max_time { dt; step; output bookkeeping }."""
+    c = ctx.cfg
+    steps = 0
+    dts = []
+    while time < c.max_time and (max_steps < 0 or steps < max_steps):
+        dt = ctx.compute_dt(time, prev_output_time)
+        ctx.time_step(dt)
+        time += dt
+        dts.append(dt)
+        steps += 1
+        if time >= prev_output_time + c.output_delta:
+            prev_output_time = time
+    return time, prev_output_time, np.array(dts)
+
+
+def probe_reconstruct(recon: str, limiter: str, stencils: np.ndarray, prec: int = FP64):
+    s = np.ascontiguousarray(stencils, dtype=np.float64).reshape(-1, 5)
+    n = s.shape[0]
+    wl = np.empty(n)
+    wr = np.empty(n)
+    _check(lib().mh_probe_reconstruct(RECONSTRUCTION[recon], SLOPE_LIMITER[limiter], prec, n, _ptr(s), _ptr(wl),
+                                      _ptr(wr)))
+    return wl, wr
+
+
+def probe_riemann(solver: str, axis: int, nd: int, gamma: float, wl: np.ndarray, wr: np.ndarray,
+                  prec: int = FP64) -> np.ndarray:
+    a = np.ascontiguousarray(wl, dtype=np.float64).reshape(-1, 2 + nd)
+    b = np.ascontiguousarray(wr, dtype=np.float64).reshape(-1, 2 + nd)
+    f = np.empty_like(a)
+    _check(lib().mh_probe_riemann(RIEMANN_SOLVER[solver], axis, nd, prec, gamma, a.shape[0], _ptr(a), _ptr(b),
+                                  _ptr(f)))
+    return f
+
+
+def probe_math(op: int, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    bb = np.ascontiguousarray(b if b is not None else a, dtype=np.float64)
+    out = np.empty_like(a)
+    _check(lib().mh_probe_math(op, a.size, _ptr(a), _ptr(bb), _ptr(out)))
+    return out

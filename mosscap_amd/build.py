@@ -1,0 +1,101 @@
+"""Build recipe for the HIP library (gfx950) and the CPU oracle.
+
+    python -m mosscap_amd.build            # both
+    python -m mosscap_amd.build --lib      # HIP library only
+
+Outputs (git-ignored, shipped to the GPU box with the tree):
+    mosscap_amd/libmosscap_hip.so   the product: kernels + C ABI (include/mosscap_hip.h)
+    oracle/_build/libmh_oracle.so   the CPU oracle (test infrastructure)
+
+Every HIP translation unit is compiled with packed-FP32 instructions
+disabled. The FP64 drop-in units use -ffp-contract=off so no multiply-add is
+fused (the reference CPU build has none); the compensated-FP32 unit allows
+contraction within an expression (-ffp-contract=on).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "mosscap_amd"
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libmosscap_hip.so"
+ORACLE_SRC = ROOT / "oracle" / "mh_oracle.cpp"
+ORACLE_LIB = ROOT / "oracle" / "_build" / "libmh_oracle.so"
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+COMMON = [
+    "-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++20", "-x", "hip",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    f"-I{ROOT / 'include'}", f"-I{CSRC}",
+]
+
+# (source, contraction mode)
+UNITS = [
+    ("kernels/stage_f64.hip", "off"),
+    ("kernels/stage_f32c.hip", "on"),
+    ("kernels/common.hip", "off"),
+    ("capi.cpp", "off"),
+]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in proc.stdout.splitlines()
+                    if "packed-fp32-ops" not in line and "is not a recognized feature" not in line)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+
+
+def build_lib(jobs: int = 4) -> Path:
+    BUILD.mkdir(parents=True, exist_ok=True)
+    objs = []
+    cmds = []
+    for src, contract in UNITS:
+        obj = BUILD / (Path(src).stem + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC, *COMMON, f"-ffp-contract={contract}", "-c", str(CSRC / src), "-o", str(obj)])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    _run([cxx, "-O3", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared",
+          str(ORACLE_SRC), "-o", str(tmp)])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true", help="HIP library only")
+    ap.add_argument("--oracle", action="store_true", help="oracle only")
+    ap.add_argument("-j", type=int, default=4)
+    a = ap.parse_args(argv)
+    if not a.oracle:
+        print("built", build_lib(a.j), flush=True)
+    if not a.lib:
+        print("built", build_oracle(), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,608 @@
+// C ABI of the hydro library (include/mosscap_hip.h): context, buffers,
+// stage sequencing, device step loop and HIP-graph replay.
+//
+// Buffer plan: three state buffers rotate through a step, so a stage never
+// writes the buffer it reads (a fused stage reads its neighbours' halo
+// cells) and the reference's Q_old copy (TimeStepping.cpp:52, 129, 226)
+// disappears: Q^n stays in its buffer for the whole step.
+#include "../../include/mosscap_hip.h"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+
+using namespace mh;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int stage_kind(TimeStepScheme ts, int stage) {
+    switch (ts) {
+        case TimeStepScheme::Rk2: return stage == 0 ? kAddFull : kRk2Final;
+        case TimeStepScheme::SspRk3: return stage == 0 ? kAddFull : stage == 1 ? kSsp3Second : kSsp3Third;
+        default: return stage == 2 ? kSsp4Third : kAddHalf;
+    }
+}
+
+bool valid_bc(int32_t b) { return b >= 0 && b <= 6; }
+
+}  // namespace
+
+struct mh_ctx {
+    int nd = 0, nvar = 0;
+    mh_grid_size sz{};
+    double dx = 0.0;
+    mh_grid_loc loc{};
+    mh_boundaries bdry{};
+    Schemes sch{};
+    double gamma = 1.4;
+    mh_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t ncells = 0, nvals = 0;
+    double* q64[3] = {nullptr, nullptr, nullptr};
+    float* qhi[3] = {nullptr, nullptr, nullptr};
+    float* qlo[3] = {nullptr, nullptr, nullptr};
+    double* staging = nullptr;  // FP64 staging for the compensated form
+    Clock* clock = nullptr;
+    unsigned int* floor_hits = nullptr;
+    int cur = 0;         // buffer holding the current state
+    int qn = 0;          // buffer holding Q^n during a step
+    int next_stage = 0;  // mh_stage bookkeeping
+    int span_h = 16;
+    int strips = 0;
+    std::string err;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    // event log for mh_run_timed: (kind, start, stop); kind 0 stage, 1 ghost, 2 clock
+    struct Ev {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev>* evlog = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+
+    hipError_t ev_get(hipEvent_t& e) {
+        if (ev_next < ev_pool.size()) {
+            e = ev_pool[ev_next++];
+            return hipSuccess;
+        }
+        hipError_t r = hipEventCreate(&e);
+        if (r == hipSuccess) {
+            ev_pool.push_back(e);
+            ev_next = ev_pool.size();
+        }
+        return r;
+    }
+    // Runs `launch` between two recorded events when timing is on.
+    template <class F>
+    hipError_t timed(int kind, F&& launch) {
+        if (!evlog) return launch();
+        Ev ev{kind, nullptr, nullptr};
+        hipError_t e = ev_get(ev.a);
+        if (e == hipSuccess) e = ev_get(ev.b);
+        if (e == hipSuccess) e = hipEventRecord(ev.a, stream);
+        if (e == hipSuccess) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(ev.b, stream);
+        if (e == hipSuccess) evlog->push_back(ev);
+        return e;
+    }
+    int64_t steps = 0;
+    double last_dt = 0.0;
+    double time = 0.0, prev_output_time = 0.0;
+
+    bool fp32c() const { return opt.precision_form == MH_FORM_FP32C; }
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return MH_ERR_HIP;
+    }
+    FormF64::Planes p64(int k) const { return FormF64::Planes{q64[k]}; }
+    FormF32c::Planes p32(int k) const { return FormF32c::Planes{qhi[k], qlo[k]}; }
+
+    GhostGrid ghost_grid(int reduce) const {
+        GhostGrid g{};
+        g.xc = sz.xc;
+        g.yc = sz.yc;
+        g.ng = sz.ng;
+        g.bc[0] = bdry.xs;
+        g.bc[1] = bdry.xe;
+        g.bc[2] = bdry.ys;
+        g.bc[3] = bdry.ye;
+        std::memcpy(g.bconst[0], bdry.xs_const, sizeof(double) * 5);
+        std::memcpy(g.bconst[1], bdry.xe_const, sizeof(double) * 5);
+        std::memcpy(g.bconst[2], bdry.ys_const, sizeof(double) * 5);
+        std::memcpy(g.bconst[3], bdry.ye_const, sizeof(double) * 5);
+        g.gamma = gamma;
+        g.reduce_dt = reduce;
+        return g;
+    }
+
+    hipError_t ghost(int in, int out, int reduce) {
+        unsigned long long* amax = &clock->amax_bits;
+        return timed(1, [&] {
+            if (fp32c()) return launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream);
+            return launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
+        });
+    }
+
+    hipError_t finalize(int advance) {
+        return timed(2, [&] { return launch_clock_finalize(clock, advance, stream); });
+    }
+
+    hipError_t cfl_all(int k) {
+        unsigned long long* amax = &clock->amax_bits;
+        if (fp32c()) return launch_cfl_f32c(nd, p32(k), (int)ncells, gamma, amax, stream);
+        return launch_cfl_f64(nd, p64(k), (int)ncells, gamma, amax, stream);
+    }
+
+    // One stage from buffer `in` into `out` with Q^n in `qn_k`, then ghost fill.
+    hipError_t stage(int in, int qn_k, int out, int kind, int reduce) {
+        hipError_t e = timed(0, [&] { return stage_kernel(in, qn_k, out, kind, reduce); });
+        if (e != hipSuccess) return e;
+        return ghost(in, out, reduce);
+    }
+
+    hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce) {
+        hipError_t e;
+        if (fp32c()) {
+            FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, (float)gamma, gamma - 1.0,
+                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor};
+            if (nd == 2) {
+                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+                e = launch_stage2d_f32c(sch, g, p, stream);
+            } else {
+                e = launch_stage1d_f32c(sch, kind, reduce, sz.xc, sz.ng, p, stream);
+            }
+        } else {
+            FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, &clock->amax_bits, nullptr};
+            if (nd == 2) {
+                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+                e = launch_stage2d_f64(sch, g, p, stream);
+            } else {
+                e = launch_stage1d_f64(sch, kind, reduce, sz.xc, sz.ng, p, stream);
+            }
+        }
+        return e;
+    }
+
+    // All stages of one step starting from `cur`; returns the new cur.
+    int enqueue_step(int k, bool reduce_last, hipError_t& e) {
+        const int ns = num_stages(sch.stepper);
+        int in = k;
+        for (int s = 0; s < ns; ++s) {
+            const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
+            e = stage(in, k, out, stage_kind(sch.stepper, s), (reduce_last && s == ns - 1) ? 1 : 0);
+            if (e != hipSuccess) return k;
+            in = out;
+        }
+        return in;
+    }
+
+    void free_all() {
+        for (int k = 0; k < 3; ++k) {
+            if (graph[k]) hipGraphExecDestroy(graph[k]);
+            graph[k] = nullptr;
+            if (q64[k]) hipFree(q64[k]);
+            if (qhi[k]) hipFree(qhi[k]);
+            if (qlo[k]) hipFree(qlo[k]);
+            q64[k] = nullptr;
+            qhi[k] = qlo[k] = nullptr;
+        }
+        for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
+        ev_pool.clear();
+        if (staging) hipFree(staging);
+        if (clock) hipFree(clock);
+        if (floor_hits) hipFree(floor_hits);
+        staging = nullptr;
+        clock = nullptr;
+        floor_hits = nullptr;
+        if (stream) hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+extern "C" {
+
+const char* mh_last_error(const mh_ctx* ctx) {
+    if (ctx == nullptr) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const mh_boundaries* bdry,
+              int32_t num_dim, int32_t num_tracers, const mh_schemes* schemes, double gamma,
+              const mh_options* opts, mh_ctx** out) {
+    g_create_error.clear();
+    auto bad = [](int code, const std::string& m) {
+        g_create_error = m;
+        return code;
+    };
+    if (!sz || !bdry || !schemes || !out) return bad(MH_ERR_INVALID, "mh_create: null argument");
+    *out = nullptr;
+    if (num_dim < 1 || num_dim > 3) {
+        return bad(MH_ERR_INVALID, "simulation.num_dim = " + std::to_string(num_dim) + ", must be in range [1, 3]");
+    }
+    if (num_dim == 3) return bad(MH_ERR_UNSUPPORTED, "3-D grids are not built in this library");
+    if (num_tracers != 0) return bad(MH_ERR_UNSUPPORTED, "tracer fields (simulation.n_extra_fields) are not built");
+    if (schemes->reconstruction < 0 || schemes->reconstruction > 3 || schemes->slope_limit < 0 ||
+        schemes->slope_limit > 2 || schemes->riemann_solver < 0 || schemes->riemann_solver > 2 ||
+        schemes->time_stepper < 0 || schemes->time_stepper > 2) {
+        return bad(MH_ERR_INVALID, "unknown scheme enum value");
+    }
+    const Reconstruction recon = (Reconstruction)schemes->reconstruction;
+    const int min_ghost = min_ghost_cells(recon);
+    if (sz->ng < min_ghost) {
+        static const char* names[] = {"fog", "muscl", "ppm", "weno5z"};
+        return bad(MH_ERR_INVALID, std::string("Insufficient ghost cells for reconstruction scheme ") +
+                                       names[schemes->reconstruction] + ", need minimum of " +
+                                       std::to_string(min_ghost) + ".");
+    }
+    const int nx = sz->xc - 2 * sz->ng;
+    const int ny = num_dim > 1 ? sz->yc - 2 * sz->ng : 1;
+    if (nx < sz->ng || ny < (num_dim > 1 ? sz->ng : 1) || sz->zc != 1 || (num_dim == 1 && sz->yc != 1)) {
+        return bad(MH_ERR_INVALID, "grid size inconsistent with num_dim / num_ghost (need n >= num_ghost)");
+    }
+    const int32_t bcs[4] = {bdry->xs, bdry->xe, bdry->ys, bdry->ye};
+    for (int k = 0; k < (num_dim > 1 ? 4 : 2); ++k) {
+        if (!valid_bc(bcs[k])) return bad(MH_ERR_INVALID, "unknown boundary type");
+    }
+    if (bdry->xs == 1 ? bdry->xe != 1 : bdry->xe == 1) {
+        return bad(MH_ERR_INVALID, "periodic boundaries must be paired (xs and xe)");
+    }
+    if (num_dim > 1 && (bdry->ys == 1 ? bdry->ye != 1 : bdry->ye == 1)) {
+        return bad(MH_ERR_INVALID, "periodic boundaries must be paired (ys and ye)");
+    }
+    if (!(dx > 0.0) || !(gamma > 1.0)) return bad(MH_ERR_INVALID, "dx must be > 0 and gamma > 1");
+    const int64_t ncells = (int64_t)sz->xc * sz->yc * sz->zc;
+    const int nvar = 2 + num_dim;
+    if (ncells * nvar >= (int64_t)1 << 31) return bad(MH_ERR_INVALID, "state larger than 2^31 values");
+
+    mh_ctx* c = new mh_ctx();
+    c->nd = num_dim;
+    c->nvar = nvar;
+    c->sz = *sz;
+    c->dx = dx;
+    if (loc) c->loc = *loc;
+    c->bdry = *bdry;
+    if (num_dim == 1) c->bdry.ys = c->bdry.ye = (int32_t)BoundaryType::Periodic;  // unused
+    c->sch = Schemes{recon, (SlopeLimiter)schemes->slope_limit, (RiemannSolver)schemes->riemann_solver,
+                     (TimeStepScheme)schemes->time_stepper};
+    c->gamma = gamma;
+    if (opts) c->opt = *opts;
+    if (c->opt.precision_form != MH_FORM_FP64 && c->opt.precision_form != MH_FORM_FP32C) {
+        delete c;
+        return bad(MH_ERR_INVALID, "unknown precision_form");
+    }
+    c->ncells = ncells;
+    c->nvals = ncells * nvar;
+    c->strips = (nx + kStripOut - 1) / kStripOut;
+    if (c->opt.span_rows > 0) {
+        c->span_h = c->opt.span_rows;
+    } else {
+        // enough wavefronts to fill 256 CUs several times over, long spans
+        // where the grid allows (the y halo costs 2R+2 rows per span)
+        c->span_h = 32;
+        while (c->span_h > 8 && (int64_t)c->strips * ((ny + c->span_h - 1) / c->span_h) < 4096) c->span_h /= 2;
+    }
+
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    const size_t bytes64 = (size_t)c->nvals * sizeof(double);
+    const size_t bytes32 = (size_t)c->nvals * sizeof(float);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        if (c->fp32c()) {
+            e = hipMalloc(&c->qhi[k], bytes32);
+            if (e == hipSuccess) e = hipMalloc(&c->qlo[k], bytes32);
+        } else {
+            e = hipMalloc(&c->q64[k], bytes64);
+        }
+    }
+    if (e == hipSuccess && c->fp32c()) e = hipMalloc(&c->staging, bytes64);
+    if (e == hipSuccess) e = hipMalloc(&c->clock, sizeof(Clock));
+    if (e == hipSuccess) e = hipMalloc(&c->floor_hits, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMemsetAsync(c->clock, 0, sizeof(Clock), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->floor_hits, 0, sizeof(unsigned int), c->stream);
+    // defined contents everywhere (ghosts of the rotating buffers included)
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        if (c->fp32c()) {
+            e = hipMemsetAsync(c->qhi[k], 0, bytes32, c->stream);
+            if (e == hipSuccess) e = hipMemsetAsync(c->qlo[k], 0, bytes32, c->stream);
+        } else {
+            e = hipMemsetAsync(c->q64[k], 0, bytes64, c->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        g_create_error = std::string("mh_create: ") + hipGetErrorString(e);
+        c->free_all();
+        delete c;
+        return MH_ERR_HIP;
+    }
+    *out = c;
+    return MH_OK;
+}
+
+void mh_destroy(mh_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    ctx->free_all();
+    delete ctx;
+}
+
+int64_t mh_num_values(const mh_ctx* ctx) { return ctx ? ctx->nvals : 0; }
+
+void* mh_stream(mh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+static int upload_impl(mh_ctx* c, const double* q, hipMemcpyKind kind) {
+    if (!c || !q) return MH_ERR_INVALID;
+    hipError_t e = hipSetDevice(c->device);
+    const size_t bytes = (size_t)c->nvals * sizeof(double);
+    if (c->fp32c()) {
+        if (e == hipSuccess) e = hipMemcpyAsync(c->staging, q, bytes, kind, c->stream);
+        if (e == hipSuccess) e = launch_split_f64_to_f32c(c->staging, c->p32(c->cur), (int)c->nvals, c->stream);
+    } else {
+        if (e == hipSuccess) e = hipMemcpyAsync(c->q64[c->cur], q, bytes, kind, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "mh_upload");
+    c->next_stage = 0;
+    return MH_OK;
+}
+
+static int download_impl(mh_ctx* c, double* q, hipMemcpyKind kind) {
+    if (!c || !q) return MH_ERR_INVALID;
+    hipError_t e = hipSetDevice(c->device);
+    const size_t bytes = (size_t)c->nvals * sizeof(double);
+    if (c->fp32c()) {
+        if (e == hipSuccess) e = launch_join_f32c_to_f64(c->p32(c->cur), c->staging, (int)c->nvals, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(q, c->staging, bytes, kind, c->stream);
+    } else {
+        if (e == hipSuccess) e = hipMemcpyAsync(q, c->q64[c->cur], bytes, kind, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "mh_download");
+    return MH_OK;
+}
+
+int mh_upload(mh_ctx* ctx, const double* q_host) { return upload_impl(ctx, q_host, hipMemcpyHostToDevice); }
+int mh_download(mh_ctx* ctx, double* q_host) { return download_impl(ctx, q_host, hipMemcpyDeviceToHost); }
+int mh_upload_device(mh_ctx* ctx, const double* q_dev) { return upload_impl(ctx, q_dev, hipMemcpyDeviceToDevice); }
+int mh_download_device(mh_ctx* ctx, double* q_dev) { return download_impl(ctx, q_dev, hipMemcpyDeviceToDevice); }
+
+int mh_fill_bcs(mh_ctx* ctx) {
+    if (!ctx) return MH_ERR_INVALID;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = ctx->ghost(ctx->cur, ctx->cur, 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_fill_bcs");
+}
+
+static hipError_t write_clock(mh_ctx* c, double max_cfl, double time, double max_time, double prev, double delta) {
+    Clock h{};
+    h.time = time;
+    h.dt = 0.0;
+    h.max_time = max_time;
+    h.prev_output_time = prev;
+    h.delta = delta;
+    h.max_cfl = max_cfl;
+    h.dx = c->dx;
+    h.amax_bits = 0ull;
+    h.step = c->steps;
+    return hipMemcpyAsync(c->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream);
+}
+
+int mh_compute_dt(mh_ctx* ctx, double max_cfl, double time, double max_time, double prev_output_time,
+                  double delta, double* dt_out) {
+    if (!ctx || !dt_out) return MH_ERR_INVALID;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = write_clock(ctx, max_cfl, time, max_time, prev_output_time, delta);
+    if (e == hipSuccess) e = ctx->cfl_all(ctx->cur);
+    if (e == hipSuccess) e = launch_clock_finalize(ctx->clock, 0, ctx->stream);
+    Clock h{};
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, ctx->clock, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_compute_dt");
+    *dt_out = h.dt;
+    return MH_OK;
+}
+
+int mh_stage(mh_ctx* ctx, int32_t stage, double dt) {
+    if (!ctx) return MH_ERR_INVALID;
+    const int ns = num_stages(ctx->sch.stepper);
+    if (stage < 0 || stage >= ns) return ctx->fail(MH_ERR_INVALID, "stage out of range for the time stepper");
+    if (stage != ctx->next_stage) return ctx->fail(MH_ERR_INVALID, "stages must run in order starting at 0");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (stage == 0) ctx->qn = ctx->cur;
+    const int in = ctx->cur;
+    const int out = (in == ctx->qn) ? (ctx->qn + 1) % 3 : 3 - in - ctx->qn;
+    if (e == hipSuccess) e = launch_clock_set_dt(ctx->clock, dt, ctx->stream);
+    if (e == hipSuccess) e = ctx->stage(in, ctx->qn, out, stage_kind(ctx->sch.stepper, stage), 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_stage");
+    ctx->cur = out;
+    ctx->next_stage = (stage + 1) % ns;
+    if (ctx->next_stage == 0) {
+        ctx->steps += 1;
+        ctx->last_dt = dt;
+    }
+    return MH_OK;
+}
+
+int mh_time_step(mh_ctx* ctx, double dt) {
+    if (!ctx) return MH_ERR_INVALID;
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is already in progress (mh_stage)");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = launch_clock_set_dt(ctx->clock, dt, ctx->stream);
+    int nc = ctx->cur;
+    if (e == hipSuccess) nc = ctx->enqueue_step(ctx->cur, false, e);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_time_step");
+    ctx->cur = nc;
+    ctx->steps += 1;
+    ctx->last_dt = dt;
+    return MH_OK;
+}
+
+int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max_time, double* prev_output_time,
+           double delta, int32_t use_graph) {
+    if (!ctx || !time || !prev_output_time || nsteps < 0) return MH_ERR_INVALID;
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is already in progress (mh_stage)");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = write_clock(ctx, max_cfl, *time, max_time, *prev_output_time, delta);
+    if (e == hipSuccess) e = ctx->cfl_all(ctx->cur);
+    for (int64_t s = 0; s < nsteps && e == hipSuccess; ++s) {
+        const int k = ctx->cur;
+        if (use_graph) {
+            if (!ctx->graph[k]) {
+                hipGraph_t g = nullptr;
+                e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) break;
+                hipError_t ce = launch_clock_finalize(ctx->clock, 1, ctx->stream);
+                if (ce == hipSuccess) ctx->enqueue_step(k, true, ce);
+                e = hipStreamEndCapture(ctx->stream, &g);
+                if (e == hipSuccess) e = ce;
+                if (e == hipSuccess) e = hipGraphInstantiate(&ctx->graph[k], g, nullptr, nullptr, 0);
+                if (g) hipGraphDestroy(g);
+                if (e != hipSuccess) break;
+            }
+            e = hipGraphLaunch(ctx->graph[k], ctx->stream);
+            ctx->cur = (k + (ctx->sch.stepper == TimeStepScheme::SspRk3 ? 1 : 2)) % 3;
+        } else {
+            e = ctx->finalize(1);
+            if (e == hipSuccess) ctx->cur = ctx->enqueue_step(k, true, e);
+        }
+        ctx->steps += 1;
+    }
+    Clock h{};
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, ctx->clock, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_run");
+    *time = h.time;
+    *prev_output_time = h.prev_output_time;
+    ctx->time = h.time;
+    ctx->prev_output_time = h.prev_output_time;
+    ctx->last_dt = h.dt;
+    return MH_OK;
+}
+
+int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max_time,
+                 double* prev_output_time, double delta, mh_timing* out) {
+    if (!ctx || !out) return MH_ERR_INVALID;
+    std::vector<mh_ctx::Ev> log;
+    log.reserve((size_t)nsteps * 10 + 4);
+    ctx->evlog = &log;
+    ctx->ev_next = 0;
+    const int rc = mh_run(ctx, nsteps, max_cfl, time, max_time, prev_output_time, delta, 0);
+    ctx->evlog = nullptr;
+    if (rc != MH_OK) return rc;
+    *out = mh_timing{};
+    hipError_t e = hipSuccess;
+    for (const auto& ev : log) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, ev.a, ev.b);
+        if (e != hipSuccess) break;
+        if (ev.kind == 0) {
+            out->stage_ms += ms;
+            out->stage_launches += 1;
+        } else if (ev.kind == 1) {
+            out->ghost_ms += ms;
+            out->ghost_launches += 1;
+        } else {
+            out->clock_ms += ms;
+            out->clock_launches += 1;
+        }
+    }
+    if (e == hipSuccess && !log.empty()) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, log.front().a, log.back().b);
+        out->step_ms = ms;
+    }
+    return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_run_timed");
+}
+
+int mh_get_stats(mh_ctx* ctx, mh_stats* out) {
+    if (!ctx || !out) return MH_ERR_INVALID;
+    unsigned int hits = 0;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMemcpyAsync(&hits, ctx->floor_hits, sizeof(hits), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_get_stats");
+    out->floor_hits = hits;
+    out->steps = ctx->steps;
+    out->last_dt = ctx->last_dt;
+    out->time = ctx->time;
+    out->prev_output_time = ctx->prev_output_time;
+    return MH_OK;
+}
+
+// ---- probes -------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+    double* p = nullptr;
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(double)); }
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+thread_local std::string g_probe_error;
+
+int probe_fail(hipError_t e) {
+    g_create_error = std::string("probe: ") + hipGetErrorString(e);
+    return MH_ERR_HIP;
+}
+
+}  // namespace
+
+int mh_probe_reconstruct(int32_t recon, int32_t limiter, int32_t prec, int64_t n, const double* stencils,
+                         double* wl, double* wr) {
+    if (n < 1 || n > (1 << 28) || !stencils || !wl || !wr || recon < 0 || recon > 3 || limiter < 0 || limiter > 2)
+        return MH_ERR_INVALID;
+    DevBuf s, l, r;
+    hipError_t e = s.alloc(5 * n);
+    if (e == hipSuccess) e = l.alloc(n);
+    if (e == hipSuccess) e = r.alloc(n);
+    if (e == hipSuccess) e = hipMemcpy(s.p, stencils, 5 * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_reconstruct(recon, limiter, prec, (int)n, s.p, l.p, r.p, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(wl, l.p, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(wr, r.p, n * sizeof(double), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MH_OK : probe_fail(e);
+}
+
+int mh_probe_riemann(int32_t solver, int32_t axis, int32_t nd, int32_t prec, double gamma, int64_t n,
+                     const double* wl, const double* wr, double* flux) {
+    if (n < 1 || n > (1 << 26) || !wl || !wr || !flux || nd < 1 || nd > 2 || axis < 0 || axis >= nd) return MH_ERR_INVALID;
+    const size_t m = (size_t)n * (2 + nd);
+    DevBuf a, b, f;
+    hipError_t e = a.alloc(m);
+    if (e == hipSuccess) e = b.alloc(m);
+    if (e == hipSuccess) e = f.alloc(m);
+    if (e == hipSuccess) e = hipMemcpy(a.p, wl, m * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.p, wr, m * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_riemann(solver, axis, nd, prec, gamma, (int)n, a.p, b.p, f.p, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(flux, f.p, m * sizeof(double), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MH_OK : probe_fail(e);
+}
+
+int mh_probe_math(int32_t op, int64_t n, const double* a, const double* b, double* out) {
+    if (n < 1 || n > (1 << 28) || !a || !out || op < 0 || op > 3 || ((op == 0 || op == 2) && !b)) return MH_ERR_INVALID;
+    DevBuf da, db, dout;
+    hipError_t e = da.alloc(n);
+    if (e == hipSuccess) e = db.alloc(n);
+    if (e == hipSuccess) e = dout.alloc(n);
+    if (e == hipSuccess) e = hipMemcpy(da.p, a, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db.p, b ? b : a, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_math(op, (int)n, da.p, db.p, dout.p, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dout.p, n * sizeof(double), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MH_OK : probe_fail(e);
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// Compensated FP32 state form (the precision form).
+//
+// * State: each conserved value is an unevaluated float pair hi + lo
+//   (|lo| <= ulp(hi)/2), stored as two float planes: 8 bytes per value, the
+//   same HBM footprint as FP64.
+// * Reconstruction and Riemann solves run in FP32 with hardware reciprocal
+//   and sqrt.
+// * cons->prim recovers the pressure p = (gamma-1)(E - |m|^2/(2 rho)) in FP64
+//   from the full hi+lo values: at high Mach number E and the kinetic energy
+//   nearly cancel, which FP32 cannot resolve.
+// * The stage combine is done in double-float arithmetic (TwoSum, and TwoProd
+//   through FMA) with double-float stage coefficients, so the state
+//   accumulates the small per-stage increments without FP32 rounding drift.
+// * The CFL signal speed is evaluated in FP64 from hi+lo.
+// * Optional positivity floors on rho and p of the reconstruction input; each
+//   interior cell and stage that needed one is counted once.
+#pragma once
+
+#include "functors.hpp"
+
+namespace mh {
+
+namespace df {  // double-float helpers (Dekker / Knuth error-free transforms)
+
+__device__ inline float2 two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (b - bb);
+    return make_float2(s, e);
+}
+__device__ inline float2 fast_two_sum(float a, float b) {  // requires |a| >= |b| or a == 0
+    const float s = a + b;
+    const float e = b - (s - a);
+    return make_float2(s, e);
+}
+__device__ inline float2 add(float2 a, float2 b) {
+    const float2 s = two_sum(a.x, b.x);
+    const float e = s.y + (a.y + b.y);
+    return fast_two_sum(s.x, e);
+}
+__device__ inline float2 add_f(float2 a, float b) {
+    const float2 s = two_sum(a.x, b);
+    const float e = s.y + a.y;
+    return fast_two_sum(s.x, e);
+}
+// a * c for a double-float constant c = (c.x, c.y)
+__device__ inline float2 mul_c(float2 a, float2 c) {
+    const float p = a.x * c.x;
+    const float e0 = __builtin_fmaf(a.x, c.x, -p);
+    const float e = e0 + (a.x * c.y + a.y * c.x);
+    return fast_two_sum(p, e);
+}
+__device__ inline float2 scale_pow2(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ inline double to_double(float2 a) { return (double)a.x + (double)a.y; }
+__device__ inline float2 from_double(double d) {
+    const float hi = (float)d;
+    const float lo = (float)(d - (double)hi);
+    return make_float2(hi, lo);
+}
+
+}  // namespace df
+
+struct FormF32c {
+    using real = float;
+    using raw = float2;
+    struct Planes {
+        float* hi;
+        float* lo;
+    };
+    struct Ptrs {
+        Planes qs, qn, out;
+        const double* dt;
+        double dx;
+        float gamma;
+        double gm1;  // gamma - 1 in FP64, for the pressure recovery
+        unsigned long long* amax_bits;
+        unsigned int* floor_hits;
+        float rho_floor;  // <= 0 disables the floors
+        float p_floor;
+    };
+
+    __device__ static inline float2 get(const Planes& a, int idx) { return make_float2(a.hi[idx], a.lo[idx]); }
+    __device__ static inline void put(const Planes& a, int idx, float2 v) {
+        a.hi[idx] = v.x;
+        a.lo[idx] = v.y;
+    }
+    __device__ static inline bool needs_qn(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 5; }
+    __device__ static inline float2 negate(float2 v) { return make_float2(-v.x, -v.y); }
+    __device__ static inline float2 from_double(double v) { return df::from_double(v); }
+
+    template <int ND, int NV, int IM>
+    __device__ static inline void diode(float2 (&q)[NV], bool start) {
+        const double mom_pre = df::to_double(q[IM]);
+        const double mom = start ? smin(mom_pre, 0.0) : smax(mom_pre, 0.0);
+        q[IM] = df::from_double(mom);
+        const double e = df::to_double(q[Cons<ND>::Ene]) - square(mom - mom_pre) / df::to_double(q[Cons<ND>::Rho]);
+        q[Cons<ND>::Ene] = df::from_double(e);
+    }
+
+    // cons->prim with FP64 pressure recovery. Returns true when a floor fired.
+    template <int ND, int NV>
+    __device__ static inline bool to_prim_nd(const Ptrs& p, const float2 (&q)[NV], float (&w)[NV]) {
+        const double rho_d = df::to_double(q[0]);
+        // 1/rho: hardware FP32 reciprocal refined by one FP64 Newton step
+        // (relative error ~1e-14, ample for the kinetic-energy subtraction).
+        const double r0 = (double)__builtin_amdgcn_rcpf((float)rho_d);
+        const double r = r0 * (2.0 - rho_d * r0);
+        double m2 = 0.0;
+        float rho = (float)rho_d;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) {
+            const double m = df::to_double(q[1 + a]);
+            m2 += m * m;
+            w[1 + a] = (float)(m * r);
+        }
+        double pres = p.gm1 * (df::to_double(q[1 + ND]) - 0.5 * m2 * r);
+        bool hit = false;
+        if (p.rho_floor > 0.0f) {
+            if (rho < p.rho_floor) {
+                rho = p.rho_floor;
+                hit = true;
+            }
+            if (pres < (double)p.p_floor) {
+                pres = (double)p.p_floor;
+                hit = true;
+            }
+        }
+        w[0] = rho;
+        w[1 + ND] = (float)pres;
+        return hit;
+    }
+
+    // The stage lambdas of TimeStepping.cpp in double-float arithmetic;
+    // src = S*dt is identically +0 and dropped. Coefficients 1/3, 2/3 and 1/6
+    // are carried as float pairs (hi + lo), exact to ~1e-16.
+    __device__ static inline float2 combine(int kind, float2 Q, float2 Qn, float dq, float /*src*/) {
+        const float2 third = make_float2(0x1.555556p-2f, -0x1.555556p-27f);
+        const float2 two_thirds = make_float2(0x1.555556p-1f, -0x1.555556p-26f);
+        const float2 sixth = make_float2(0x1.555556p-3f, -0x1.555556p-28f);
+        switch (kind) {
+            case 0: return df::add_f(Q, dq);
+            case 1: return df::scale_pow2(df::add_f(df::add(Qn, Q), dq), 0.5f);
+            case 2: return df::add(df::mul_c(Qn, make_float2(0.75f, 0.0f)), df::scale_pow2(df::add_f(Q, dq), 0.25f));
+            case 3: return df::add(df::mul_c(Qn, third), df::mul_c(df::add_f(Q, dq), two_thirds));
+            case 4: return df::add_f(Q, 0.5f * dq);
+            default: {
+                const float2 a = df::add(df::mul_c(Qn, two_thirds), df::mul_c(Q, third));
+                return df::add(a, df::mul_c(make_float2(dq, 0.0f), sixth));
+            }
+        }
+    }
+
+    template <int ND, int NV>
+    __device__ static inline double signal_speed_nd(double gamma, const float2 (&q)[NV]) {
+        double qd[NV], w[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qd[v] = df::to_double(q[v]);
+        cons_to_prim<ND>(gamma, qd, w);
+        return mh::signal_speed<ND>(gamma, w);
+    }
+};
+
+}  // namespace mh

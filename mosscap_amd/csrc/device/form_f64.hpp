@@ -1,0 +1,69 @@
+// FP64 state form: the drop-in. State and arithmetic are double, every
+// expression keeps the reference's grouping (compile with -ffp-contract=off).
+#pragma once
+
+#include "functors.hpp"
+
+namespace mh {
+
+struct FormF64 {
+    using real = double;
+    using raw = double;
+    struct Planes {
+        double* p;  // [var][k][j][i]
+    };
+    struct Ptrs {
+        Planes qs;         // Q^(s): input of the stage, ghosts filled
+        Planes qn;         // Q^n: state at the start of the step
+        Planes out;        // Q^(s+1)
+        const double* dt;  // device copy of the step's dt
+        double dx;
+        double gamma;
+        unsigned long long* amax_bits;  // CFL max of c_s + |v| (bit pattern)
+        unsigned int* floor_hits;       // unused in this form
+    };
+
+    __device__ static inline double get(const Planes& a, int idx) { return a.p[idx]; }
+    __device__ static inline void put(const Planes& a, int idx, double v) { a.p[idx] = v; }
+    __device__ static inline bool needs_qn(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 5; }
+    __device__ static inline double negate(double v) { return -v; }
+    __device__ static inline double from_double(double v) { return v; }
+
+    // SymmetricOutflowDiode momentum clamp and energy correction,
+    // Boundaries.hpp:96-106.
+    template <int ND, int NV, int IM>
+    __device__ static inline void diode(double (&q)[NV], bool start) {
+        const double mom_pre = q[IM];
+        q[IM] = start ? smin(mom_pre, 0.0) : smax(mom_pre, 0.0);
+        q[Cons<ND>::Ene] -= square(q[IM] - mom_pre) / q[Cons<ND>::Rho];
+    }
+
+    template <int ND, int NV>
+    __device__ static inline bool to_prim_nd(const Ptrs& p, const double (&q)[NV], double (&w)[NV]) {
+        cons_to_prim<ND>(p.gamma, q, w);
+        return false;  // no floors in the drop-in
+    }
+
+    // TimeStepping.cpp stage lambdas (RK2 60-67/80-87, SSPRK3 137-144/157-164/
+    // 177-184, SSPRK(4,3) 234-301), grouped exactly as written there.
+    __device__ static inline double combine(int kind, double Q, double Qn, double dq, double src) {
+        switch (kind) {
+            case 0: return Q + (dq + src);
+            case 1: return 0.5 * (Qn + Q + dq + src);
+            case 2: return 0.75 * Qn + 0.25 * (Q + dq + src);
+            case 3: return (1.0 / 3.0) * Qn + (2.0 / 3.0) * (Q + dq + src);
+            case 4: return Q + 0.5 * (dq + src);
+            default: return (2.0 / 3.0) * Qn + (1.0 / 3.0) * Q + (1.0 / 6.0) * (dq + src);
+        }
+    }
+
+    // dt_reducer, Hydro.cpp:192-206: cons_to_prim then c_s + |v|.
+    template <int ND, int NV>
+    __device__ static inline double signal_speed_nd(double gamma, const double (&q)[NV]) {
+        double w[NV];
+        cons_to_prim<ND>(gamma, q, w);
+        return mh::signal_speed<ND>(gamma, w);
+    }
+};
+
+}  // namespace mh

@@ -1,0 +1,297 @@
+// Fused SSPRK stage for 2-D unsplit finite volumes: the "marching strip".
+//
+// One wavefront owns a strip of 62 output columns and marches down a span of
+// rows. Lane t holds column c = i0 - 1 + t (the 62 outputs are lanes 1..62,
+// lanes 0 and 63 supply the outer reconstructions and faces). Per output row j:
+//
+//   1. load row j+1+R of Q^(s) (coalesced, x fastest), cons->prim, write the
+//      primitive row into a 4-row LDS ring (x stencils) and append the lane's
+//      own column value to a (2R+1)-row register window (y stencil);
+//   2. y sweep: reconstruct cell j+1 from the register window, Riemann flux on
+//      face j+1 from (RR of cell j, RL of cell j+1), dy = Gy_j - Gy_{j+1};
+//   3. x sweep: reconstruct row j from the LDS ring, take RR of the left
+//      neighbour lane, Riemann flux on the face, dx = Fx_c - Fx_{c+1} (flux of
+//      the right neighbour lane);
+//   4. combine in registers: dq = (0 + dx) + dy, dq *= dt/dx, stage update with
+//      Q^(s) and Q^n, store Q^(s+1); optionally fold |v|+c_s into a running
+//      max for the next CFL time step.
+//
+// HBM traffic per cell and stage is one read of Q^(s) (plus a 6-column /
+// 6-row halo, mostly served from L2), one read of Q^n for stages >= 1 and one
+// write of Q^(s+1). R is the stencil order of the reconstruction (FOG 0,
+// MUSCL 1, PPM and WENO5-Z 2).
+//
+// The arithmetic follows Mosscap's unfused sequence (Hydro.cpp:8-166,
+// TimeStepping.cpp:9-35 and the stage lambdas): both sweeps read the same
+// primitive field (unsplit), the x difference is added before the y
+// difference, and the stage formulas keep their grouping. The state form
+// (`Form`) supplies loads, cons->prim, the stage combine and stores, so the
+// same march serves the FP64 drop-in and the compensated FP32 form.
+#pragma once
+
+#include "functors.hpp"
+#include "wave.hpp"
+
+namespace mh {
+
+constexpr int kStripOut = 62;  // output columns per wavefront
+
+// Stage update kinds: the lambdas of TimeStepping.cpp, src = S*dt with S = 0.
+enum StageKind : int32_t {
+    kAddFull = 0,      // Q + (dq + src)                       RK2/SSPRK3 stage 0
+    kRk2Final = 1,     // 0.5*(((Qn + Q) + dq) + src)          RK2 stage 1
+    kSsp3Second = 2,   // 0.75*Qn + 0.25*((Q + dq) + src)      SSPRK3 stage 1
+    kSsp3Third = 3,    // (1/3)*Qn + (2/3)*((Q + dq) + src)    SSPRK3 stage 2
+    kAddHalf = 4,      // Q + 0.5*(dq + src)                   SSPRK(4,3) stages 0, 1, 3
+    kSsp4Third = 5,    // ((2/3)*Qn + (1/3)*Q) + (1/6)*(dq + src)   SSPRK(4,3) stage 2
+};
+
+struct StageGrid {
+    int32_t xc, yc, ng, nx, ny;
+    int32_t strips, span_h;
+    int32_t kind;        // StageKind
+    int32_t reduce_dt;   // fold the new state into the CFL max
+};
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__device__ inline void stage2d_march(const StageGrid& g, const typename Form::Ptrs& p) {
+    using T = typename Form::real;
+    using Raw = typename Form::raw;
+    constexpr int NV = 4;  // 2-D ideal gas: rho, mx, my, E
+    constexpr int ORD = stencil_order(R);
+    constexpr int WIN = 2 * ORD + 1;
+    constexpr int ROW = kWave + 2 * ORD;
+    constexpr int RING = 4;  // rows j .. j+1+ORD are live
+    static_assert(ORD + 2 <= RING, "ring too shallow");
+
+    __shared__ T ring[RING][NV][ROW];
+
+    const int lane = threadIdx.x;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const int strip = (int)(b % (uint32_t)g.strips);
+    const int span = (int)(b / (uint32_t)g.strips);
+    const int i0 = g.ng + strip * kStripOut;
+    const int j0 = g.ng + span * g.span_h;
+    const int j1 = min(j0 + g.span_h, g.ng + g.ny);
+    const int c = i0 - 1 + lane;
+    const int c_ld = min(c, g.xc - 1);
+    const int xc = g.xc;
+    const int plane = g.xc * g.yc;
+
+    // Halo lanes: lanes [0, ORD) fill entries [0, ORD) (columns left of the
+    // strip), lanes [ORD, 2*ORD) fill entries [64+ORD, 64+2*ORD).
+    const bool halo_lane = lane < 2 * ORD;
+    const int h_entry = lane < ORD ? lane : kWave + lane;
+    const int h_col = min(i0 - 1 - ORD + h_entry, g.xc - 1);
+
+    const T gamma = p.gamma;
+    const EosT<T> eos{gamma};
+    const T dt = (T)p.dt[0];
+    const T dtdx = dt / (T)p.dx;
+    const T src = T(0) * dt;
+
+    T win[WIN][NV];
+    const bool out_lane = lane >= 1 && lane <= kStripOut && c < g.ng + g.nx;
+    unsigned int floor_hits = 0;
+
+    auto commit_row = [&](int r, const Raw (&q)[NV], const Raw (&qh)[NV]) {
+        const int slot = r & (RING - 1);
+        T w[NV];
+        const bool hit = Form::template to_prim_nd<2>(p, q, w);
+        // each interior cell is committed by exactly one (strip, span) owner
+        floor_hits += (hit && out_lane && r >= j0 && r < j1) ? 1u : 0u;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ring[slot][v][lane + ORD] = w[v];
+#pragma unroll
+        for (int k = 0; k + 1 < WIN; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) win[k][v] = win[k + 1][v];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) win[WIN - 1][v] = w[v];
+        if (ORD > 0 && halo_lane) {
+            T wh[NV];
+            Form::template to_prim_nd<2>(p, qh, wh);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ring[slot][v][h_entry] = wh[v];
+        }
+    };
+    auto fetch_row = [&](int r, Raw (&q)[NV], Raw (&qh)[NV]) {
+        const int base = r * xc;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = Form::get(p.qs, v * plane + base + c_ld);
+        if (ORD > 0 && halo_lane) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qh[v] = Form::get(p.qs, v * plane + base + h_col);
+        }
+    };
+    auto recon_y = [&](T (&wl)[NV], T (&wr)[NV]) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            auto s = [&](int o) -> T { return win[ORD + o][v]; };
+            reconstruct_stencil<R, SL, T>(s, wl[v], wr[v]);
+        }
+    };
+
+    // ---- prologue: rows j0-1-ORD .. j0+ORD; y faces up to j0 ------------------
+    Raw qa[NV], qha[NV];
+    T yRp[NV], Gyp[NV];
+    {
+        Raw qb[NV], qhb[NV];
+        for (int r = j0 - 1 - ORD; r <= j0 - 1 + ORD; ++r) {
+            fetch_row(r, qb, qhb);
+            commit_row(r, qb, qhb);
+        }
+        T yl[NV], yr[NV];
+        recon_y(yl, yr);  // cell j0-1
+#pragma unroll
+        for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
+        fetch_row(j0 + ORD, qb, qhb);
+        commit_row(j0 + ORD, qb, qhb);
+        recon_y(yl, yr);  // cell j0
+        riemann_flux<RS, 1, 2>(eos, yRp, yl, Gyp);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
+        if (j0 < j1) fetch_row(j0 + 1 + ORD, qa, qha);
+    }
+
+    double amax = 0.0;
+
+    for (int j = j0; j < j1; ++j) {
+        // Stage-combine operands for this row, issued early.
+        Raw qs_c[NV], qn_c[NV];
+        const int cell = j * xc + c_ld;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            qs_c[v] = Form::get(p.qs, v * plane + cell);
+            qn_c[v] = Form::needs_qn(g.kind) ? Form::get(p.qn, v * plane + cell) : qs_c[v];
+        }
+        wave_lds_sync();  // earlier ring reads complete before the slot is reused
+        commit_row(j + 1 + ORD, qa, qha);
+        if (j + 1 < j1) fetch_row(j + 2 + ORD, qa, qha);  // prefetch the next row
+
+        // y sweep: cell j+1, face j+1
+        T yl[NV], yr[NV], Gy[NV], dy[NV];
+        recon_y(yl, yr);
+        riemann_flux<RS, 1, 2>(eos, yRp, yl, Gy);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            dy[v] = Gyp[v] - Gy[v];
+            Gyp[v] = Gy[v];
+            yRp[v] = yr[v];
+        }
+
+        wave_lds_sync();  // row j of the ring visible to every lane
+        // x sweep on row j
+        const int slot = j & (RING - 1);
+        T xl[NV], xr[NV], xrm[NV], Fx[NV], Fxp[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            auto s = [&](int o) -> T { return ring[slot][v][lane + ORD + o]; };
+            reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
+        riemann_flux<RS, 0, 2>(eos, xrm, xl, Fx);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
+
+        if (out_lane) {
+            Raw qo[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                T q_update = T(0);
+                q_update += Fx[v] - Fxp[v];
+                q_update += dy[v];
+                q_update *= dtdx;
+                qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, src);
+                Form::put(p.out, v * plane + j * xc + c, qo[v]);
+            }
+            if (g.reduce_dt) {
+                const double a = Form::template signal_speed_nd<2>((double)p.gamma, qo);
+                amax = a > amax ? a : amax;
+            }
+        }
+    }
+
+    if (g.reduce_dt) {
+        const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
+        if (lane == 0) atomicMax((unsigned long long*)p.amax_bits, (unsigned long long)m);
+    }
+    if (p.floor_hits != nullptr) {
+        const unsigned long long any = __ballot(floor_hits != 0u);
+        if (any != 0ull) {
+            const unsigned int total = wave_sum_u32(floor_hits);
+            if (lane == 0) atomicAdd(p.floor_hits, total);
+        }
+    }
+}
+
+// 1-D stage: one wavefront covers 62 output cells of a (segment of a) row;
+// lanes are cells c = i0 - 1 + t, the halo entries come from lanes [0, 2R).
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, int reduce_dt, int base, int xc,
+                                       int ng, int nx, int i0, const typename Form::real dtdx,
+                                       const typename Form::real src, typename Form::real* lds_row /*[3][64+2R]*/,
+                                       double& amax, unsigned int& floor_hits) {
+    using T = typename Form::real;
+    using Raw = typename Form::raw;
+    constexpr int NV = 3;  // rho, mx, E
+    constexpr int ORD = stencil_order(R);
+    constexpr int ROW = kWave + 2 * ORD;
+    const int lane = threadIdx.x & 63;
+    const int c = i0 - 1 + lane;
+    const int c_ld = min(c, xc - 1);
+    const bool halo_lane = lane < 2 * ORD;
+    const int h_entry = lane < ORD ? lane : kWave + lane;
+    const int h_col = min(i0 - 1 - ORD + h_entry, xc - 1);
+    const EosT<T> eos{p.gamma};
+
+    Raw q[NV], qn[NV], qh[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        q[v] = Form::get(p.qs, base + v * xc + c_ld);
+        qn[v] = Form::needs_qn(kind) ? Form::get(p.qn, base + v * xc + c_ld) : q[v];
+        if (ORD > 0 && halo_lane) qh[v] = Form::get(p.qs, base + v * xc + h_col);
+    }
+    T w[NV];
+    const bool out_lane = lane >= 1 && lane <= kStripOut && c < ng + nx;
+    floor_hits += (Form::template to_prim_nd<1>(p, q, w) && out_lane) ? 1u : 0u;
+    wave_lds_sync();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lds_row[v * ROW + lane + ORD] = w[v];
+    if (ORD > 0 && halo_lane) {
+        T wh[NV];
+        Form::template to_prim_nd<1>(p, qh, wh);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds_row[v * ROW + h_entry] = wh[v];
+    }
+    wave_lds_sync();
+    T xl[NV], xr[NV], xrm[NV], Fx[NV], Fxp[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        auto s = [&](int o) -> T { return lds_row[v * ROW + lane + ORD + o]; };
+        reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
+    riemann_flux<RS, 0, 1>(eos, xrm, xl, Fx);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
+    if (out_lane) {
+        Raw qo[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T q_update = T(0);
+            q_update += Fx[v] - Fxp[v];
+            q_update *= dtdx;
+            qo[v] = Form::combine(kind, q[v], qn[v], q_update, src);
+            Form::put(p.out, base + v * xc + c, qo[v]);
+        }
+        if (reduce_dt) {
+            const double a = Form::template signal_speed_nd<1>((double)p.gamma, qo);
+            amax = a > amax ? a : amax;
+        }
+    }
+}
+
+}  // namespace mh

@@ -1,0 +1,16 @@
+// Fused stage kernels of the FP64 drop-in path. Built with -ffp-contract=off:
+// no multiply-add contraction, so results match the reference CPU build bitwise.
+#include "stage_inst.hpp"
+
+namespace mh {
+
+hipError_t launch_stage2d_f64(const Schemes& s, const StageGrid& g, const FormF64::Ptrs& p, hipStream_t st) {
+    return launch_stage2d_impl<FormF64>(s, g, p, st);
+}
+
+hipError_t launch_stage1d_f64(const Schemes& s, int kind, int reduce_dt, int xc, int ng, const FormF64::Ptrs& p,
+                              hipStream_t st) {
+    return launch_stage1d_impl<FormF64>(s, kind, reduce_dt, xc, ng, p, st);
+}
+
+}  // namespace mh

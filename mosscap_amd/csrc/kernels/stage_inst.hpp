@@ -1,0 +1,109 @@
+// Kernel entry points and scheme dispatch for the fused stages, instantiated
+// once per state form (stage_f64.hip, stage_f32c.hip).
+#pragma once
+
+#include "../launch.hpp"
+
+namespace mh {
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__global__ __launch_bounds__(64) void stage2d_kernel(StageGrid g, typename Form::Ptrs p) {
+    stage2d_march<Form, R, SL, RS>(g, p);
+}
+
+struct Stage1dGrid {
+    int32_t xc, ng, nx, kind, reduce_dt;
+};
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__global__ __launch_bounds__(64) void stage1d_kernel(Stage1dGrid g, typename Form::Ptrs p) {
+    using T = typename Form::real;
+    constexpr int ORD = stencil_order(R);
+    __shared__ T lds[3 * (kWave + 2 * ORD)];
+    const int i0 = g.ng + (int)blockIdx.x * kStripOut;
+    const T dt = (T)p.dt[0];
+    const T dtdx = dt / (T)p.dx;
+    const T src = T(0) * dt;
+    double amax = 0.0;
+    unsigned int hits = 0;
+    stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, 0, g.xc, g.ng, g.nx, i0, dtdx, src, lds, amax, hits);
+    if (g.reduce_dt) {
+        const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
+        if (threadIdx.x == 0) atomicMax(p.amax_bits, (unsigned long long)m);
+    }
+    if (p.floor_hits != nullptr) {
+        const unsigned int total = wave_sum_u32(hits);
+        if (threadIdx.x == 0 && total != 0u) atomicAdd(p.floor_hits, total);
+    }
+}
+
+template <class Form>
+using Stage2dFn = void (*)(StageGrid, typename Form::Ptrs);
+template <class Form>
+using Stage1dFn = void (*)(Stage1dGrid, typename Form::Ptrs);
+
+template <class Form, Reconstruction R, SlopeLimiter SL>
+struct SolverPick {
+    static Stage2dFn<Form> two(RiemannSolver rs) {
+        switch (rs) {
+            case RiemannSolver::Rusanov: return stage2d_kernel<Form, R, SL, RiemannSolver::Rusanov>;
+            case RiemannSolver::Hll: return stage2d_kernel<Form, R, SL, RiemannSolver::Hll>;
+            default: return stage2d_kernel<Form, R, SL, RiemannSolver::Hllc>;
+        }
+    }
+    static Stage1dFn<Form> one(RiemannSolver rs) {
+        switch (rs) {
+            case RiemannSolver::Rusanov: return stage1d_kernel<Form, R, SL, RiemannSolver::Rusanov>;
+            case RiemannSolver::Hll: return stage1d_kernel<Form, R, SL, RiemannSolver::Hll>;
+            default: return stage1d_kernel<Form, R, SL, RiemannSolver::Hllc>;
+        }
+    }
+};
+
+// (reconstruction, limiter) -> instance; FOG and WENO5-Z ignore the limiter
+// (is_limiter_agnostic, Reconstruct.hpp:34-45).
+template <class Form, class Pick>
+auto pick_scheme(const Schemes& s, Pick pick) {
+    using SL = SlopeLimiter;
+    using RC = Reconstruction;
+    switch (s.recon) {
+        case RC::Fog: return pick.template operator()<RC::Fog, SL::MonotonizedCentral>();
+        case RC::Weno5Z: return pick.template operator()<RC::Weno5Z, SL::MonotonizedCentral>();
+        case RC::Muscl:
+            if (s.limiter == SL::VanLeer) return pick.template operator()<RC::Muscl, SL::VanLeer>();
+            if (s.limiter == SL::Minmod) return pick.template operator()<RC::Muscl, SL::Minmod>();
+            return pick.template operator()<RC::Muscl, SL::MonotonizedCentral>();
+        default:
+            if (s.limiter == SL::VanLeer) return pick.template operator()<RC::Ppm, SL::VanLeer>();
+            if (s.limiter == SL::Minmod) return pick.template operator()<RC::Ppm, SL::Minmod>();
+            return pick.template operator()<RC::Ppm, SL::MonotonizedCentral>();
+    }
+}
+
+template <class Form>
+hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typename Form::Ptrs& p, hipStream_t st) {
+    if (g.nx < 1 || g.ny < 1 || g.ng < min_ghost_cells(s.recon) || g.xc != g.nx + 2 * g.ng ||
+        g.yc != g.ny + 2 * g.ng || g.span_h < 1 || g.strips != (g.nx + kStripOut - 1) / kStripOut) {
+        return hipErrorInvalidValue;
+    }
+    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() { return SolverPick<Form, R, SL>::two(s.solver); };
+    Stage2dFn<Form> fn = pick_scheme<Form>(s, pick);
+    const int spans = (g.ny + g.span_h - 1) / g.span_h;
+    hipLaunchKernelGGL(fn, dim3((unsigned)(g.strips * spans)), dim3(kWave), 0, st, g, p);
+    return hipGetLastError();
+}
+
+template <class Form>
+hipError_t launch_stage1d_impl(const Schemes& s, int kind, int reduce_dt, int xc, int ng,
+                               const typename Form::Ptrs& p, hipStream_t st) {
+    const int nx = xc - 2 * ng;
+    if (nx < 1 || ng < min_ghost_cells(s.recon)) return hipErrorInvalidValue;
+    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() { return SolverPick<Form, R, SL>::one(s.solver); };
+    Stage1dFn<Form> fn = pick_scheme<Form>(s, pick);
+    const Stage1dGrid g{xc, ng, nx, kind, reduce_dt};
+    const int segs = (nx + kStripOut - 1) / kStripOut;
+    hipLaunchKernelGGL(fn, dim3((unsigned)segs), dim3(kWave), 0, st, g, p);
+    return hipGetLastError();
+}
+
+}  // namespace mh

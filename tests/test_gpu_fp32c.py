@@ -1,0 +1,109 @@
+"""The compensated FP32 precision form against the FP64 path (= oracle).
+
+Tolerances (rel L1 per conserved variable, interior cells): density <= 1e-5
+on the KH workload (the BASELINE target); every variable <= 5e-5 elsewhere.
+"""
+import numpy as np
+import pytest
+
+import mosscap_amd as M
+from mosscap_amd import Config, HydroContext, problems, probe_reconstruct, probe_riemann
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l1(a, b, cfg):
+    g = cfg.ng
+    if cfg.num_dim > 1:
+        a, b = a[:, 0, g:g + cfg.ny, g:g + cfg.nx], b[:, 0, g:g + cfg.ny, g:g + cfg.nx]
+    else:
+        a, b = a[:, 0, 0, g:g + cfg.nx], b[:, 0, 0, g:g + cfg.nx]
+    a = a.reshape(a.shape[0], -1)
+    b = b.reshape(b.shape[0], -1)
+    return np.abs(a - b).sum(1) / np.abs(b).sum(1)
+
+
+def _pair(cfg, q0, nsteps, **kw):
+    out = []
+    for form in (M.FP64, M.FP32C):
+        with HydroContext(cfg, precision_form=form, **kw) as ctx:
+            ctx.upload(q0)
+            ctx.fill_bcs()
+            ctx.run(nsteps)
+            out.append(ctx.download())
+            st = ctx.stats()
+    return out[0], out[1], st
+
+
+def test_upload_download_round_trip():
+    cfg = problems.kh_config(64)
+    q0 = problems.kh_band(cfg)
+    with HydroContext(cfg, precision_form=M.FP32C) as ctx:
+        ctx.upload(q0)
+        q = ctx.download()
+    # hi + lo carries ~48 bits
+    np.testing.assert_allclose(q, q0, rtol=2e-14, atol=0)
+
+
+def test_kh_512_200_steps_rel_l1():
+    cfg = problems.kh_config(512)
+    q0 = problems.kh_band(cfg)
+    q64, q32, _ = _pair(cfg, q0, 200)
+    r = rel_l1(q32, q64, cfg)
+    print("KH 512^2 200 steps FP32C vs FP64 rel L1:", r)
+    assert r[0] <= 1e-5, r
+    assert (r <= 1e-4).all(), r
+
+
+@pytest.mark.parametrize("recon,lim,solver,stepper", [("muscl", "minmod", "hllc", "rk2"),
+                                                      ("ppm", "monotonizedcentral", "hll", "ssprk3"),
+                                                      ("fog", "minmod", "rusanov", "ssprk4")])
+def test_other_schemes(recon, lim, solver, stepper):
+    cfg = problems.kh_config(128)
+    cfg.reconstruction, cfg.slope_limiter, cfg.riemann_solver, cfg.time_stepper = recon, lim, solver, stepper
+    q0 = problems.kh_band(cfg)
+    q64, q32, _ = _pair(cfg, q0, 50)
+    r = rel_l1(q32, q64, cfg)
+    assert (r <= 5e-5).all(), r
+
+
+def test_sod_256_fp32c():
+    cfg = problems.sod_config()
+    q0 = problems.shock_tube_2d(cfg)
+    q64, q32, st = _pair(cfg, q0, 100)
+    r = rel_l1(q32, q64, cfg)
+    assert (r <= 5e-5).all(), r
+
+
+def test_floors_count_and_hold():
+    """Sedov-type blast at small size with floors on: no NaN, hits counted."""
+    cfg = problems.sedov_config(128)
+    q0 = problems.sedov(cfg)
+    with HydroContext(cfg, precision_form=M.FP32C, rho_floor=1e-8, p_floor=1e-9) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        ctx.run(60)
+        q = ctx.download()
+        st = ctx.stats()
+    assert np.isfinite(q).all()
+    assert st.floor_hits >= 0
+
+
+def test_fp32_functors_close_to_fp64(orc):
+    rng = np.random.default_rng(1)
+    s = rng.uniform(0.5, 2.0, size=(20000, 5))
+    for recon, lim in [("weno5z", "minmod"), ("ppm", "monotonizedcentral"), ("muscl", "vanleer")]:
+        wl, wr = probe_reconstruct(recon, lim, s, prec=M.FP32C)
+        ol, orr = orc.reconstruct(recon, lim, s)
+        np.testing.assert_allclose(wl, ol, rtol=2e-5, atol=2e-6)
+        np.testing.assert_allclose(wr, orr, rtol=2e-5, atol=2e-6)
+    w = np.empty((20000, 4))
+    w[:, 0] = rng.uniform(0.5, 2, 20000)
+    w[:, 1:3] = rng.normal(size=(20000, 2))
+    w[:, 3] = rng.uniform(0.5, 2, 20000)
+    w2 = w.copy()
+    w2[:, 0] *= rng.uniform(0.8, 1.2, 20000)
+    for solver in ("rusanov", "hll", "hllc"):
+        f = probe_riemann(solver, 0, 2, 1.4, w, w2, prec=M.FP32C)
+        fo = orc.riemann(solver, 0, 2, 1.4, w, w2)
+        np.testing.assert_allclose(f, fo, rtol=1e-4, atol=1e-5)

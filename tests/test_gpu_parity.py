@@ -1,0 +1,283 @@
+"""Parity of the HIP drop-in (FP64) with the oracle, through the C ABI.
+
+The bar for the FP64 path is bitwise equality (which implies the rel L1
+<= 1e-12 per conserved variable the drop-in promises); every test that compares
+states also reports the rel L1.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from mosscap_amd import Config, HydroContext, problems, probe_math, probe_reconstruct, probe_riemann, run_to_time
+import mosscap_amd as M
+
+pytestmark = pytest.mark.gpu
+
+G = 1.4
+
+
+def rel_l1(a, b, cfg):
+    g = cfg.ng
+    if cfg.num_dim > 1:
+        a, b = a[:, 0, g:g + cfg.ny, g:g + cfg.nx], b[:, 0, g:g + cfg.ny, g:g + cfg.nx]
+    else:
+        a, b = a[:, 0, 0, g:g + cfg.nx], b[:, 0, 0, g:g + cfg.nx]
+    a = a.reshape(a.shape[0], -1)
+    b = b.reshape(b.shape[0], -1)
+    return np.abs(a - b).sum(1) / np.maximum(np.abs(b).sum(1), 1e-300)
+
+
+def assert_bitwise(gpu, ref, cfg, what=""):
+    r = rel_l1(gpu, ref, cfg)
+    assert (r <= 1e-12).all(), f"{what} rel L1 {r}"
+    diff = np.argwhere(gpu != ref)
+    assert diff.size == 0, f"{what}: {len(diff)} values differ, first {diff[:5].tolist()}, rel L1 {r}"
+
+
+# ---- functors -----------------------------------------------------------------
+def test_fp64_divide_and_sqrt_are_ieee():
+    rng = np.random.default_rng(11)
+    a = np.concatenate([rng.uniform(-10, 10, 200000), 10.0 ** rng.uniform(-300, 300, 50000),
+                        [1.0, 2.0, 3.0, 1e-310, 5e-324, 1.7976931348623157e308]])
+    b = np.concatenate([rng.uniform(-10, 10, 200000), 10.0 ** rng.uniform(-300, 300, 50000),
+                        [3.0, 7.0, 9.0, 3.0, 2.0, 0.5]])
+    b[b == 0] = 1.0
+    np.testing.assert_array_equal(probe_math(0, a, b), a / b)
+    s = np.abs(a)
+    np.testing.assert_array_equal(probe_math(1, s), np.sqrt(s))
+
+
+RECONS = [("fog", "minmod"), ("muscl", "vanleer"), ("muscl", "monotonizedcentral"), ("muscl", "minmod"),
+          ("ppm", "vanleer"), ("ppm", "monotonizedcentral"), ("ppm", "minmod"), ("weno5z", "minmod")]
+
+
+def _stencils(n, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.normal(size=(n, 5))
+    s[: n // 4] = np.round(s[: n // 4] * 2) / 2          # ties, zeros, flat runs
+    s[n // 4: n // 2] = np.cumsum(rng.uniform(0, 1, (n // 4, 5)), axis=1)  # monotone
+    s[n // 2: n // 2 + 100] = 1.25                         # uniform
+    return s
+
+
+@pytest.mark.parametrize("recon,lim", RECONS)
+def test_reconstruct_bitwise(orc, recon, lim):
+    s = _stencils(100000, 5)
+    wl, wr = probe_reconstruct(recon, lim, s)
+    ol, or_ = orc.reconstruct(recon, lim, s)
+    np.testing.assert_array_equal(wl, ol)
+    np.testing.assert_array_equal(wr, or_)
+
+
+def _prim_states(n, nd, seed):
+    rng = np.random.default_rng(seed)
+    w = np.empty((n, 2 + nd))
+    w[:, 0] = 10 ** rng.uniform(-1, 1, n)
+    w[:, 1:1 + nd] = rng.normal(size=(n, nd))
+    w[:, 1 + nd] = 10 ** rng.uniform(-2, 2, n)
+    w[: n // 10, 1:1 + nd] = 0.0
+    return w
+
+
+@pytest.mark.parametrize("solver", ["rusanov", "hll", "hllc"])
+@pytest.mark.parametrize("nd,axis", [(1, 0), (2, 0), (2, 1)])
+def test_riemann_bitwise(orc, solver, nd, axis):
+    wl = _prim_states(50000, nd, 1)
+    wr = _prim_states(50000, nd, 2)
+    wr[:5000] = wl[:5000]  # consistency branch
+    np.testing.assert_array_equal(probe_riemann(solver, axis, nd, G, wl, wr), orc.riemann(solver, axis, nd, G, wl, wr))
+
+
+# ---- ghost fill ----------------------------------------------------------------
+KINDS = ["wall", "periodic", "symmetric", "symmetricoutflowdiode", "zerograd", "constant"]
+
+
+@pytest.mark.parametrize("ng", [1, 2, 3])
+def test_ghost_fill_bitwise(orc, ng):
+    rng = np.random.default_rng(100 + ng)
+    pairs = [(a, b) for a in KINDS for b in KINDS if (a == "periodic") == (b == "periodic")]
+    for k, (xs, xe) in enumerate(pairs):
+        ys, ye = pairs[(k * 5 + 1) % len(pairs)]
+        cfg = Config(num_dim=2, nx=9, ny=7, ng=ng, reconstruction="fog", bc=(xs, xe, ys, ye),
+                     bc_const=tuple(tuple(rng.uniform(0.5, 2.0, 5)) for _ in range(4)))
+        q = rng.uniform(0.5, 2.0, size=cfg.shape)
+        q[1:3] -= 1.25
+        sim = orc.OracleSim(cfg)
+        sim.set_q(q)
+        sim.fill_bcs()
+        with HydroContext(cfg) as ctx:
+            ctx.upload(q)
+            ctx.fill_bcs()
+            np.testing.assert_array_equal(ctx.download(), sim.get_q(), err_msg=str(cfg.bc))
+
+
+def test_ghost_fill_1d_bitwise(orc):
+    rng = np.random.default_rng(7)
+    for xs, xe in [("wall", "zerograd"), ("periodic", "periodic"), ("symmetricoutflowdiode", "constant")]:
+        cfg = Config(num_dim=1, nx=11, ng=3, reconstruction="weno5z", bc=(xs, xe, "wall", "wall"),
+                     bc_const=((1.0, 0.5, 3.0, 0, 0), (2.0, -0.5, 4.0, 0, 0), (0,) * 5, (0,) * 5))
+        q = rng.uniform(0.5, 2.0, size=cfg.shape)
+        sim = orc.OracleSim(cfg)
+        sim.set_q(q)
+        sim.fill_bcs()
+        with HydroContext(cfg) as ctx:
+            ctx.upload(q)
+            ctx.fill_bcs()
+            np.testing.assert_array_equal(ctx.download(), sim.get_q())
+
+
+# ---- full steps -------------------------------------------------------------------
+def _kh_small(nx, ny, recon, lim, solver, stepper, ng=3, bc=("periodic",) * 4):
+    cfg = Config(num_dim=2, nx=nx, ny=ny, ng=ng, bc=bc, reconstruction=recon, slope_limiter=lim,
+                 riemann_solver=solver, time_stepper=stepper, max_cfl=0.4, max_time=1.0, output_delta=0.1)
+    return cfg
+
+
+def _host_run(cfg, q0, nsteps, orc):
+    sim = orc.OracleSim(cfg)
+    sim.set_q(q0)
+    sim.fill_bcs()
+    t_o, p_o, dts_o = sim.run(max_steps=nsteps)
+    with HydroContext(cfg) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        t_g, p_g, dts_g = run_to_time(ctx, max_steps=nsteps)
+        qg = ctx.download()
+    return qg, sim.get_q(), (t_g, p_g, dts_g), (t_o, p_o, dts_o)
+
+
+COMBOS = list(itertools.product(["fog", "muscl", "ppm", "weno5z"], ["rusanov", "hll", "hllc"],
+                                ["rk2", "ssprk3", "ssprk4"]))
+
+
+@pytest.mark.parametrize("recon,solver,stepper", COMBOS)
+def test_step_parity_all_schemes(orc, recon, solver, stepper):
+    lim = {"fog": "minmod", "muscl": "vanleer", "ppm": "monotonizedcentral", "weno5z": "minmod"}[recon]
+    # 130 x 45 interior: three strips (the last partial) and ragged spans
+    cfg = _kh_small(130, 45, recon, lim, solver, stepper, ng=3,
+                    bc=("wall", "zerograd", "periodic", "periodic"))
+    q0 = problems.kh_band(cfg, seed=3)
+    qg, qo, hg, ho = _host_run(cfg, q0, 4, orc)
+    np.testing.assert_array_equal(hg[2], ho[2])
+    assert_bitwise(qg, qo, cfg, f"{recon}/{solver}/{stepper}")
+
+
+@pytest.mark.parametrize("lim", ["vanleer", "monotonizedcentral", "minmod"])
+@pytest.mark.parametrize("recon", ["muscl", "ppm"])
+def test_step_parity_limiters(orc, recon, lim):
+    cfg = _kh_small(64, 70, recon, lim, "hllc", "ssprk3", ng=3, bc=("symmetric", "wall", "zerograd", "wall"))
+    q0 = problems.kh_band(cfg, seed=9)
+    qg, qo, _, _ = _host_run(cfg, q0, 5, orc)
+    assert_bitwise(qg, qo, cfg, f"{recon}/{lim}")
+
+
+@pytest.mark.parametrize("recon,solver,stepper", [("weno5z", "hllc", "ssprk3"), ("muscl", "hll", "rk2"),
+                                                  ("ppm", "rusanov", "ssprk4")])
+def test_step_parity_1d(orc, recon, solver, stepper):
+    cfg = Config(num_dim=1, nx=300, ng=3, bc=("zerograd", "wall", "wall", "wall"), reconstruction=recon,
+                 slope_limiter="minmod", riemann_solver=solver, time_stepper=stepper, max_cfl=0.4,
+                 max_time=0.2, output_delta=0.1)
+    x, _ = problems.cell_centres(cfg)
+    rho = np.where(x < 0.4, 1.0, 0.125)[None, :]
+    p = np.where(x < 0.4, 1.0, 0.1)[None, :]
+    q0 = problems.prim_to_cons(cfg, rho, 0.3 + 0 * rho, None, p)
+    qg, qo, hg, ho = _host_run(cfg, q0, 30, orc)
+    np.testing.assert_array_equal(hg[2], ho[2])
+    assert_bitwise(qg, qo, cfg, "1d")
+
+
+def test_sod_256_to_t_end(orc):
+    """2-D Sod 256^2, MUSCL-minmod + HLLC + RK2, CFL 0.4, t_end 0.2."""
+    cfg = problems.sod_config()
+    q0 = problems.shock_tube_2d(cfg)
+    sim = orc.OracleSim(cfg)
+    sim.set_q(q0)
+    sim.fill_bcs()
+    t_o, p_o, dts_o = sim.run()
+    with HydroContext(cfg) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        t_g, p_g, dts_g = run_to_time(ctx)
+        qg = ctx.download()
+    assert t_g == t_o and len(dts_g) == len(dts_o)
+    print(f"Sod 256^2: {len(dts_g)} steps to t={t_g}, rel L1 {rel_l1(qg, sim.get_q(), cfg)}")
+    assert_bitwise(qg, sim.get_q(), cfg, "sod")
+
+
+def test_kh_256_weno_hllc_ssprk3(orc):
+    cfg = problems.kh_config(256)
+    q0 = problems.kh_band(cfg)
+    qg, qo, hg, ho = _host_run(cfg, q0, 20, orc)
+    np.testing.assert_array_equal(hg[2], ho[2])
+    assert_bitwise(qg, qo, cfg, "kh256")
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_device_loop_equals_host_loop(orc, use_graph):
+    """mh_run (dt fused into the last stage, optional graph replay) == compute_dt + time_step."""
+    cfg = _kh_small(100, 64, "weno5z", "minmod", "hllc", "ssprk3", bc=("periodic", "periodic", "wall", "constant"))
+    cfg.bc_const = ((0,) * 5, (0,) * 5, (0,) * 5, (1.0, 0.3, -0.2, 9.0, 0))
+    q0 = problems.kh_band(cfg, seed=5)
+    for stepper in ("rk2", "ssprk3", "ssprk4"):
+        cfg.time_stepper = stepper
+        with HydroContext(cfg) as a, HydroContext(cfg) as b:
+            a.upload(q0)
+            a.fill_bcs()
+            b.upload(q0)
+            b.fill_bcs()
+            t_h, p_h, _ = run_to_time(a, max_steps=7)
+            t_d, p_d = b.run(7, use_graph=use_graph)
+            assert (t_d, p_d) == (t_h, p_h)
+            np.testing.assert_array_equal(b.download(), a.download())
+            # and on through the output-time clamp at t = 0.1
+            t_d, p_d = b.run(400, time=t_d, prev_output_time=p_d, use_graph=use_graph)
+            t_h, p_h, _ = run_to_time(a, time=t_h, prev_output_time=p_h, max_steps=400)
+            assert (t_d, p_d) == (t_h, p_h)
+            np.testing.assert_array_equal(b.download(), a.download())
+
+
+def test_stagewise_parity(orc):
+    cfg = _kh_small(80, 50, "weno5z", "minmod", "hllc", "ssprk3")
+    q0 = problems.kh_band(cfg, seed=2)
+    sim = orc.OracleSim(cfg)
+    sim.set_q(q0)
+    sim.fill_bcs()
+    with HydroContext(cfg) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        dt = ctx.compute_dt(0.0, 0.0)
+        assert dt == sim.compute_dt(0.0, 0.0)
+        sim.begin_step()
+        for s in range(3):
+            if s > 0:
+                sim.fill_bcs()
+            sim.stage(s, dt)
+            ctx.stage(s, dt)
+            qo = sim.get_q()
+            sim_q = qo.copy()
+            sim.fill_bcs()
+            assert_bitwise(ctx.download(), sim.get_q(), cfg, f"stage {s}")
+            sim.set_q(sim_q)
+
+
+def test_kh_2048_full_size_one_step(orc):
+    """BASELINE config 2 at full size: one SSPRK3 step bitwise against the oracle,
+    then 10 more device steps conserve mass, momentum and energy (periodic)."""
+    cfg = problems.kh_config(2048)
+    q0 = problems.kh_band(cfg)
+    sim = orc.OracleSim(cfg)
+    sim.set_q(q0)
+    sim.fill_bcs()
+    sim.run(max_steps=1)
+    with HydroContext(cfg) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        t, p = ctx.run(1)
+        q1 = ctx.download()
+        assert_bitwise(q1, sim.get_q(), cfg, "kh2048 step 1")
+        g = cfg.ng
+        tot0 = q1[:, 0, g:-g, g:-g].sum(axis=(1, 2))
+        ctx.run(10, time=t, prev_output_time=p)
+        tot = ctx.download()[:, 0, g:-g, g:-g].sum(axis=(1, 2))
+        np.testing.assert_allclose(tot, tot0, rtol=1e-12, atol=1e-12 * np.abs(tot0).max())
