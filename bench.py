@@ -55,15 +55,15 @@ def _rel_l1(a, b, g):
     return (np.abs(a - b).sum(1) / np.abs(b).sum(1)).tolist()
 
 
-def time_form(M, cfg, q0, form, steps, warmup):
-    """Graph-replayed steps: wall time of exactly `steps` steps after `warmup`."""
-    ctx = M.HydroContext(cfg, precision_form=form)
+def time_form(M, cfg, q0, form, steps, warmup, use_graph=True, span_rows=0):
+    """Steps replayed from a graph: wall time of exactly `steps` steps after `warmup`."""
+    ctx = M.HydroContext(cfg, precision_form=form, span_rows=span_rows)
     ctx.upload(q0)
     ctx.fill_bcs()
-    t, p = ctx.run(warmup, use_graph=True)
+    t, p = ctx.run(warmup, use_graph=use_graph)
     _sync()
     t0 = time.perf_counter()
-    t, p = ctx.run(steps, time=t, prev_output_time=p, use_graph=True)
+    t, p = ctx.run(steps, time=t, prev_output_time=p, use_graph=use_graph)
     _sync()
     elapsed = time.perf_counter() - t0
     # per-kernel HIP-event timing on the library's stream (separate pass)
@@ -112,6 +112,20 @@ def cpu_baseline(cfg, q0, seconds):
                       f"{el:.1f} s on {threads} threads of {cpu}"}
 
 
+def copy_probe(mib: int) -> int:
+    """A plain device-to-device copy kernel of known size, 5 times (torch's
+    elementwise copy), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,7 +135,13 @@ def main(argv=None):
     ap.add_argument("--parity-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (profiler passes)")
+    ap.add_argument("--span-rows", type=int, default=0, help="rows per wavefront (0: tuned default)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
     a = ap.parse_args(argv)
+    if a.copy_probe_mb:
+        return copy_probe(a.copy_probe_mb)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; a multi-GPU figure is not part of it "
               "(halo-exchange sharding is described in DESIGN.md)", file=sys.stderr)
@@ -134,8 +154,8 @@ def main(argv=None):
     q0 = problems.kh_band(cfg)
     cells = cfg.nx * cfg.ny
 
-    el64, tm64 = time_form(M, cfg, q0, M.FP64, a.steps, a.warmup)
-    el32, tm32 = time_form(M, cfg, q0, M.FP32C, a.steps, a.warmup)
+    el64, tm64 = time_form(M, cfg, q0, M.FP64, a.steps, a.warmup, not a.no_graph, a.span_rows)
+    el32, tm32 = time_form(M, cfg, q0, M.FP32C, a.steps, a.warmup, not a.no_graph, a.span_rows)
     v64 = cells * a.steps / el64 / 1e6
     v32 = cells * a.steps / el32 / 1e6
     rel = parity(M, cfg, q0, a.parity_steps) if a.parity_steps > 0 else None
