@@ -126,6 +126,55 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+TRAFFIC_JSON = ROOT / "profiles" / "stage2d_kh2048_traffic.json"
+
+
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_bytes, out_json):
+    """Per-dispatch HBM traffic of each kernel from two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), scaled by what the same
+    counters read for a plain copy of `copy_bytes` bytes."""
+    import csv
+    from collections import defaultdict
+
+    def load(path):
+        d = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            d[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)
+        return d
+
+    cf, cw = load(copy_fetch_csv), load(copy_write_csv)
+    big_f = max(cf.items(), key=lambda kv: max(kv[1]))[1]
+    big_w = max(cw.items(), key=lambda kv: max(kv[1]))[1]
+    kf = copy_bytes / (sum(big_f) / len(big_f))
+    kw = copy_bytes / (sum(big_w) / len(big_w))
+    f, w = load(fetch_csv), load(write_csv)
+    rows = {}
+    for name in f:
+        if name in w:
+            rf = sum(f[name]) / len(f[name]) * kf
+            rw = sum(w[name]) / len(w[name]) * kw
+            rows[name] = {"dispatches": len(f[name]), "read_bytes": rf, "write_bytes": rw, "traffic": rf + rw}
+    out = {"fetch_scale": kf, "write_scale": kw, "kernels": rows,
+           "sources": [str(fetch_csv), str(write_csv), str(copy_fetch_csv), str(copy_write_csv)]}
+    Path(out_json).write_text(json.dumps(out, indent=1))
+    print(json.dumps({"fetch_scale": kf, "write_scale": kw}))
+    for n, r in rows.items():
+        print(f"{r['dispatches']:4d} read {r['read_bytes'] / 1e6:10.2f} MB  write {r['write_bytes'] / 1e6:10.2f} MB  {n[:110]}")
+    return 0
+
+
+def measured_traffic(kernel_prefix):
+    """Average HBM bytes per launch of the matching kernel from the committed PMC summary."""
+    try:
+        data = json.loads(TRAFFIC_JSON.read_text())
+    except (OSError, ValueError):
+        return None
+    vals = [r for n, r in data["kernels"].items() if n.startswith(kernel_prefix)]
+    if not vals:
+        return None
+    return sum(v["traffic"] for v in vals) / len(vals)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +188,12 @@ def main(argv=None):
     ap.add_argument("--span-rows", type=int, default=0, help="rows per wavefront (0: tuned default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="summarise rocprofv3 --pmc CSVs into the traffic JSON (CPU only) and exit")
+    ap.add_argument("--copy-bytes", type=int, default=512 << 20)
     a = ap.parse_args(argv)
+    if a.pmc_summary:
+        return pmc_summary(*a.pmc_summary, a.copy_bytes, TRAFFIC_JSON)
     if a.copy_probe_mb:
         return copy_probe(a.copy_probe_mb)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -194,7 +248,9 @@ def main(argv=None):
                                                             "rho_mx_my_E": [float(f"{r:.3e}") for r in rel]},
         "adjoint_over_forward": None,
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("void mh::stage2d_kernel<mh::FormF32c, (mh::Reconstruction)3, "
+                                                 "(mh::SlopeLimiter)1, (mh::RiemannSolver)2>"),
                      "kernel": "stage2d_kernel<FormF32c, Weno5Z, HLLC>",
                      "avg_launch_ms": round(avg32, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "kernel_split_ms_per_step": {
