@@ -55,11 +55,16 @@ def _rel_l1(a, b, g):
     return (np.abs(a - b).sum(1) / np.abs(b).sum(1)).tolist()
 
 
-def time_form(M, cfg, q0, form, steps, warmup, use_graph=True, span_rows=0):
-    """Steps replayed from a graph: wall time of exactly `steps` steps after `warmup`."""
+def time_form(M, cfg, q0, form, steps, warmup, use_graph=True, span_rows=0, autotune=True):
+    """Steps replayed from a graph: wall time of exactly `steps` steps after `warmup`
+    (the span height is picked by the library's autotuner unless given)."""
     ctx = M.HydroContext(cfg, precision_form=form, span_rows=span_rows)
     ctx.upload(q0)
     ctx.fill_bcs()
+    tune = None
+    if autotune and not span_rows:
+        best, table = ctx.autotune(steps=3)
+        tune = {"span_rows": best, "ms_per_step": {k: round(v, 4) for k, v in table.items()}}
     t, p = ctx.run(warmup, use_graph=use_graph)
     _sync()
     t0 = time.perf_counter()
@@ -69,7 +74,7 @@ def time_form(M, cfg, q0, form, steps, warmup, use_graph=True, span_rows=0):
     # per-kernel HIP-event timing on the library's stream (separate pass)
     t, p, tm = ctx.run_timed(steps, time=t, prev_output_time=p)
     ctx.close()
-    return elapsed, tm
+    return elapsed, tm, tune
 
 
 def parity(M, cfg, q0, nsteps):
@@ -185,7 +190,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (profiler passes)")
-    ap.add_argument("--span-rows", type=int, default=0, help="rows per wavefront (0: tuned default)")
+    ap.add_argument("--span-rows", type=int, default=0, help="rows per wavefront (0: autotuned)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the built-in span heuristic")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
@@ -208,8 +214,10 @@ def main(argv=None):
     q0 = problems.kh_band(cfg)
     cells = cfg.nx * cfg.ny
 
-    el64, tm64 = time_form(M, cfg, q0, M.FP64, a.steps, a.warmup, not a.no_graph, a.span_rows)
-    el32, tm32 = time_form(M, cfg, q0, M.FP32C, a.steps, a.warmup, not a.no_graph, a.span_rows)
+    el64, tm64, tune64 = time_form(M, cfg, q0, M.FP64, a.steps, a.warmup, not a.no_graph, a.span_rows,
+                                   not a.no_autotune)
+    el32, tm32, tune32 = time_form(M, cfg, q0, M.FP32C, a.steps, a.warmup, not a.no_graph, a.span_rows,
+                                   not a.no_autotune)
     v64 = cells * a.steps / el64 / 1e6
     v32 = cells * a.steps / el32 / 1e6
     rel = parity(M, cfg, q0, a.parity_steps) if a.parity_steps > 0 else None
@@ -253,6 +261,7 @@ def main(argv=None):
                                                  "(mh::SlopeLimiter)1, (mh::RiemannSolver)2>"),
                      "kernel": "stage2d_kernel<FormF32c, Weno5Z, HLLC>",
                      "avg_launch_ms": round(avg32, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "autotune": {"fp32c": tune32, "fp64": tune64},
         "kernel_split_ms_per_step": {
             "stage": round(tm32.stage_ms / a.steps, 4), "ghost": round(tm32.ghost_ms / a.steps, 4),
             "clock": round(tm32.clock_ms / a.steps, 4)},

@@ -138,6 +138,14 @@ typedef struct {
 int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max_time,
                  double* prev_output_time, double delta, mh_timing* out);
 
+/* Autotuner (no counterpart in the reference): times `steps` device steps of
+ * the current state for each candidate span height (rows marched per
+ * wavefront in 2-D; 0-terminated list, NULL = built-in candidates) with HIP
+ * events, keeps the fastest for this context and returns it in *best_span
+ * (and the per-candidate milliseconds per step in ms_out when non-NULL). The
+ * state is restored afterwards. */
+int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* best_span, double* ms_out);
+
 int mh_get_stats(mh_ctx* ctx, mh_stats* out);
 
 /* HIP stream the context launches on (a hipStream_t), for event timing. */

@@ -16,7 +16,8 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "libmosscap_hip.so"
+# MOSSCAP_HIP_LIB selects an alternative build of the same library (tuning experiments)
+LIB_PATH = Path(os.environ.get("MOSSCAP_HIP_LIB", str(_HERE / "libmosscap_hip.so")))
 
 # Enum values, as in the reference's name tables (Reconstruct.hpp:14-31,
 # Riemann.hpp:15-19, TimeStepping.hpp:15-19, State.hpp:71-79).
@@ -98,6 +99,7 @@ _SIGNATURES = [
                          C.POINTER(C.c_double), C.c_double, C.c_int32]),
     ("mh_run_timed", C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_double), C.c_double,
                                C.POINTER(C.c_double), C.c_double, C.POINTER(MhTiming)]),
+    ("mh_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     ("mh_get_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
     ("mh_stream", C.c_void_p, [C.c_void_p]),
     ("mh_probe_reconstruct", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
@@ -274,6 +276,14 @@ class HydroContext:
         self._c(lib().mh_run_timed(self._h, nsteps, c.max_cfl, C.byref(t), c.max_time, C.byref(p),
                                    c.output_delta, C.byref(tm)))
         return t.value, p.value, tm
+
+    def autotune(self, steps: int = 3, candidates=None):
+        """Pick the span height (rows per wavefront) by timing; returns (best, {span: ms/step})."""
+        cand = np.array(list(candidates or [4, 8, 12, 16, 24, 32, 48, 64]) + [0], dtype=np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int32()
+        self._c(lib().mh_autotune(self._h, steps, _ptr(cand), C.byref(best), _ptr(ms)))
+        return best.value, {int(c): float(m) for c, m in zip(cand[:-1], ms[:-1]) if m > 0}
 
     def stats(self) -> MhStats:
         s = MhStats()

@@ -10,7 +10,8 @@ Outputs (git-ignored, shipped to the GPU box with the tree):
 Every HIP translation unit is compiled with packed-FP32 instructions
 disabled. The FP64 drop-in units use -ffp-contract=off so no multiply-add is
 fused (the reference CPU build has none); the compensated-FP32 unit allows
-contraction within an expression (-ffp-contract=on).
+contraction (-ffp-contract=fast) except inside its error-free transforms,
+which pin contraction off with a pragma.
 """
 from __future__ import annotations
 
@@ -43,7 +44,7 @@ COMMON = [
 # (source, contraction mode)
 UNITS = [
     ("kernels/stage_f64.hip", "off"),
-    ("kernels/stage_f32c.hip", "on"),
+    ("kernels/stage_f32c.hip", "fast"),
     ("kernels/common.hip", "off"),
     ("capi.cpp", "off"),
 ]
@@ -57,21 +58,26 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
 
 
-def build_lib(jobs: int = 4) -> Path:
-    BUILD.mkdir(parents=True, exist_ok=True)
+def build_lib(jobs: int = 4, defines: list[str] | None = None, out: Path | None = None) -> Path:
+    """Compile the HIP units and link the shared library (`defines`: extra -D
+    knobs, e.g. MH_STAGE_WAVES_PER_EU=2, for experiments built to `out`)."""
+    lib = Path(out) if out else LIB
+    build = BUILD if out is None else lib.parent / (lib.stem + "_obj")
+    build.mkdir(parents=True, exist_ok=True)
+    extra = [f"-D{d}" for d in (defines or [])]
     objs = []
     cmds = []
     for src, contract in UNITS:
-        obj = BUILD / (Path(src).stem + ".o")
+        obj = build / (Path(src).stem + ".o")
         objs.append(obj)
-        cmds.append([HIPCC, *COMMON, f"-ffp-contract={contract}", "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([HIPCC, *COMMON, *extra, f"-ffp-contract={contract}", "-c", str(CSRC / src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for f in [ex.submit(_run, c) for c in cmds]:
             f.result()
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_oracle() -> Path:
@@ -89,9 +95,11 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--lib", action="store_true", help="HIP library only")
     ap.add_argument("--oracle", action="store_true", help="oracle only")
     ap.add_argument("-j", type=int, default=4)
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor knob")
+    ap.add_argument("--out", default=None, help="library path (experiments)")
     a = ap.parse_args(argv)
     if not a.oracle:
-        print("built", build_lib(a.j), flush=True)
+        print("built", build_lib(a.j, a.defines, a.out), flush=True)
     if not a.lib:
         print("built", build_oracle(), flush=True)
     return 0

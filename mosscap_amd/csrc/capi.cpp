@@ -527,6 +527,89 @@ int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, doub
     return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_run_timed");
 }
 
+int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* best_span, double* ms_out) {
+    if (!ctx || !best_span || steps < 1) return MH_ERR_INVALID;
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
+    static const int32_t kDefault[] = {4, 8, 12, 16, 24, 32, 48, 64, 0};
+    const int32_t* cand = candidates ? candidates : kDefault;
+    hipError_t e = hipSetDevice(ctx->device);
+    // keep the state in a scratch copy and restore it after each trial
+    void* keep_a = nullptr;
+    void* keep_b = nullptr;
+    const size_t bytes = ctx->fp32c() ? ctx->nvals * sizeof(float) : ctx->nvals * sizeof(double);
+    if (e == hipSuccess) e = hipMalloc(&keep_a, bytes);
+    if (e == hipSuccess && ctx->fp32c()) e = hipMalloc(&keep_b, bytes);
+    const int start = ctx->cur;
+    auto stash = [&](bool restore) -> hipError_t {
+        void* d0 = ctx->fp32c() ? (void*)ctx->qhi[ctx->cur] : (void*)ctx->q64[ctx->cur];
+        if (restore) {
+            hipError_t r = hipMemcpyAsync(d0, keep_a, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+            if (r == hipSuccess && ctx->fp32c())
+                r = hipMemcpyAsync(ctx->qlo[ctx->cur], keep_b, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+            return r;
+        }
+        hipError_t r = hipMemcpyAsync(keep_a, d0, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        if (r == hipSuccess && ctx->fp32c())
+            r = hipMemcpyAsync(keep_b, ctx->qlo[ctx->cur], bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        return r;
+    };
+    if (e == hipSuccess) e = stash(false);
+    const int old_span = ctx->span_h;
+    const int64_t old_steps = ctx->steps;
+    double best = 1e300;
+    int best_h = old_span;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    const int ny = ctx->nd > 1 ? ctx->sz.yc - 2 * ctx->sz.ng : 1;
+    for (int k = 0; cand[k] > 0 && e == hipSuccess; ++k) {
+        if (cand[k] > ny && k > 0) continue;
+        ctx->span_h = cand[k];
+        for (int g = 0; g < 3; ++g) {  // graphs bake the launch shape in
+            if (ctx->graph[g]) hipGraphExecDestroy(ctx->graph[g]);
+            ctx->graph[g] = nullptr;
+        }
+        double t = 0.0, p = 0.0;
+        int rc = mh_run(ctx, 1, 0.4, &t, 1e30, &p, 1e30, 1);  // warm-up and graph capture
+        if (rc == MH_OK) e = hipEventRecord(a, ctx->stream);
+        if (rc == MH_OK && e == hipSuccess) rc = mh_run(ctx, steps, 0.4, &t, 1e30, &p, 1e30, 1);
+        if (rc != MH_OK) {
+            e = hipErrorUnknown;
+            break;
+        }
+        e = hipEventRecord(b, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        const double per = ms / steps;
+        if (ms_out) ms_out[k] = per;
+        if (per < best) {
+            best = per;
+            best_h = cand[k];
+        }
+        ctx->cur = start;
+        if (e == hipSuccess) e = stash(true);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (keep_a) hipFree(keep_a);
+    if (keep_b) hipFree(keep_b);
+    for (int g = 0; g < 3; ++g) {
+        if (ctx->graph[g]) hipGraphExecDestroy(ctx->graph[g]);
+        ctx->graph[g] = nullptr;
+    }
+    ctx->steps = old_steps;
+    ctx->cur = start;
+    if (e != hipSuccess) {
+        ctx->span_h = old_span;
+        return ctx->hip_fail(e, "mh_autotune");
+    }
+    ctx->span_h = best_h;
+    *best_span = best_h;
+    return MH_OK;
+}
+
 int mh_get_stats(mh_ctx* ctx, mh_stats* out) {
     if (!ctx || !out) return MH_ERR_INVALID;
     unsigned int hits = 0;

@@ -20,31 +20,38 @@
 
 namespace mh {
 
+// The error-free transforms below must not be contracted into FMAs (the
+// translation unit otherwise allows contraction), hence the pragmas.
 namespace df {  // double-float helpers (Dekker / Knuth error-free transforms)
 
 __device__ inline float2 two_sum(float a, float b) {
+#pragma clang fp contract(off)
     const float s = a + b;
     const float bb = s - a;
     const float e = (a - (s - bb)) + (b - bb);
     return make_float2(s, e);
 }
 __device__ inline float2 fast_two_sum(float a, float b) {  // requires |a| >= |b| or a == 0
+#pragma clang fp contract(off)
     const float s = a + b;
     const float e = b - (s - a);
     return make_float2(s, e);
 }
 __device__ inline float2 add(float2 a, float2 b) {
+#pragma clang fp contract(off)
     const float2 s = two_sum(a.x, b.x);
     const float e = s.y + (a.y + b.y);
     return fast_two_sum(s.x, e);
 }
 __device__ inline float2 add_f(float2 a, float b) {
+#pragma clang fp contract(off)
     const float2 s = two_sum(a.x, b);
     const float e = s.y + a.y;
     return fast_two_sum(s.x, e);
 }
 // a * c for a double-float constant c = (c.x, c.y)
 __device__ inline float2 mul_c(float2 a, float2 c) {
+#pragma clang fp contract(off)
     const float p = a.x * c.x;
     const float e0 = __builtin_fmaf(a.x, c.x, -p);
     const float e = e0 + (a.x * c.y + a.y * c.x);

@@ -18,6 +18,8 @@
 #include <cstdint>
 
 #include "layout.hpp"
+#include "fast32.hpp"
+#include <type_traits>
 
 namespace mh {
 
@@ -191,6 +193,8 @@ __host__ __device__ inline void reconstruct_stencil(const S& s, T& wL, T& wR) {
         }
         wL = l;
         wR = r;
+    } else if constexpr (std::is_same_v<T, float>) {  // precision form
+        weno5z_f32(s, wL, wR);
     } else {  // WENO5-Z (Borges et al. 2008), 217-274
         constexpr T b0c = T(13) / T(12);
         constexpr T b1c = T(0.25);
@@ -283,6 +287,8 @@ __host__ __device__ inline void riemann_flux(const EosT<T>& eos, const WL& wL, c
         for (int i = 0; i < n_hydro; ++i) {
             flux[i] = sM * (sR * fL[i] - sL * fR[i] + sR * sL * (qR[i] - qL[i]));
         }
+    } else if constexpr (std::is_same_v<T, float>) {  // precision form
+        hllc_f32<Axis, NumDim>(gamma, wL, wR, flux);
     } else {  // HLLC, 75-176 (Athena-style pressure-based wave speeds)
         const T csL = sound_speed<NumDim>(gamma, wL);
         const T csR = sound_speed<NumDim>(gamma, wR);

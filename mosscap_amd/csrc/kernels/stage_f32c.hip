@@ -1,6 +1,10 @@
 // Fused stage kernels of the compensated FP32 precision form. Built with
-// -ffp-contract=fast: multiply-adds may fuse (the error-free transforms of
-// the combine contain no multiply-add that contraction could change).
+// -ffp-contract=fast: multiply-adds may fuse, except inside the error-free
+// transforms of the stage combine (form_f32c.hpp pins those with a pragma).
+#ifndef MH_STAGE_WAVES_F32C
+#define MH_STAGE_WAVES_F32C 3
+#endif
+#define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F32C
 #include "stage_inst.hpp"
 
 namespace mh {

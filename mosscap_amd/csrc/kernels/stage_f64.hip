@@ -1,5 +1,9 @@
 // Fused stage kernels of the FP64 drop-in path. Built with -ffp-contract=off:
 // no multiply-add contraction, so results match the reference CPU build bitwise.
+#ifndef MH_STAGE_WAVES_F64
+#define MH_STAGE_WAVES_F64 1
+#endif
+#define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F64
 #include "stage_inst.hpp"
 
 namespace mh {

@@ -4,10 +4,17 @@
 
 #include "../launch.hpp"
 
+// Occupancy target of the fused stage (waves per SIMD the register allocator
+// must allow); each form's translation unit sets its own value.
+#ifndef MH_STAGE_WAVES_PER_EU
+#define MH_STAGE_WAVES_PER_EU 1
+#endif
+
 namespace mh {
 
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
-__global__ __launch_bounds__(64) void stage2d_kernel(StageGrid g, typename Form::Ptrs p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MH_STAGE_WAVES_PER_EU)))
+void stage2d_kernel(StageGrid g, typename Form::Ptrs p) {
     stage2d_march<Form, R, SL, RS>(g, p);
 }
 

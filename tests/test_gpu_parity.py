@@ -281,3 +281,20 @@ def test_kh_2048_full_size_one_step(orc):
         ctx.run(10, time=t, prev_output_time=p)
         tot = ctx.download()[:, 0, g:-g, g:-g].sum(axis=(1, 2))
         np.testing.assert_allclose(tot, tot0, rtol=1e-12, atol=1e-12 * np.abs(tot0).max())
+
+
+def test_autotune_keeps_state_and_results(orc):
+    """The autotuner restores the state; results do not depend on the span height."""
+    cfg = _kh_small(300, 200, "weno5z", "minmod", "hllc", "ssprk3")
+    q0 = problems.kh_band(cfg, seed=4)
+    with HydroContext(cfg) as a, HydroContext(cfg, span_rows=5) as b:
+        a.upload(q0)
+        a.fill_bcs()
+        best, table = a.autotune(steps=2, candidates=[4, 8, 16, 64])
+        assert best in (4, 8, 16, 64) and len(table) == 4
+        np.testing.assert_array_equal(a.download()[:, :, 3:-3, 3:-3], q0[:, :, 3:-3, 3:-3])
+        b.upload(q0)
+        b.fill_bcs()
+        a.run(5)
+        b.run(5)
+        np.testing.assert_array_equal(a.download(), b.download())
