@@ -151,6 +151,41 @@ int mh_get_stats(mh_ctx* ctx, mh_stats* out);
 /* HIP stream the context launches on (a hipStream_t), for event timing. */
 void* mh_stream(mh_ctx* ctx);
 
+/* ---- batched ensemble of independent 1-D problems -------------------------
+ * No counterpart in the reference (one Simulation per process there). Each
+ * problem is a 1-D grid of lengths[p] interior cells on [0, x_dim] with ng
+ * ghosts, sharing schemes, gamma and the x boundaries of bdry; problem p's
+ * state is the block Q_p[3][lengths[p] + 2 ng] at offset offsets[p] of one
+ * packed array. Each problem keeps its own dt (CFL over its own cells,
+ * Hydro.cpp:208-242) and time. */
+typedef struct mh_batch mh_batch;
+
+int mh_batch_create(int32_t nprob, const int32_t* lengths, int32_t ng, double x_dim, const mh_boundaries* bdry,
+                    const mh_schemes* schemes, double gamma, const mh_options* opts, mh_batch** out);
+void mh_batch_destroy(mh_batch* b);
+const char* mh_batch_last_error(const mh_batch* b);
+int64_t mh_batch_num_values(const mh_batch* b);
+/* Element offset of every problem's block (nprob values). */
+int mh_batch_offsets(const mh_batch* b, int64_t* offsets);
+/* Move the packed state (ghosts included); upload resets every problem's time to 0. */
+int mh_batch_upload(mh_batch* b, const double* q_host);
+int mh_batch_download(mh_batch* b, double* q_host);
+int mh_batch_fill_bcs(mh_batch* b);
+/* nsteps steps of every problem. mode 0: one persistent launch per stage
+ * covering all problems (work list of 62-cell segments); mode 1: one launch
+ * per problem and stage (the comparison baseline). With use_graph, each step
+ * replays a captured HIP graph. Steps past max_time have dt = 0. */
+int mh_batch_run(mh_batch* b, int64_t nsteps, double max_cfl, double max_time, int32_t mode, int32_t use_graph);
+/* Per-problem clock and last dt (nprob values each; either may be NULL). */
+int mh_batch_times(mh_batch* b, double* times, double* last_dt);
+/* Persistent grid size in wavefronts (0 = built-in default), and an
+ * autotuner that times candidate sizes (0-terminated list of waves per CU,
+ * NULL = built-in) over `steps` steps, restores the state and keeps the best. */
+int mh_batch_set_waves(mh_batch* b, int32_t waves);
+int mh_batch_autotune(mh_batch* b, int32_t steps, const int32_t* waves_per_cu, int32_t* best_waves, double* ms_out);
+void* mh_batch_stream(mh_batch* b);
+int mh_batch_stats(mh_batch* b, mh_stats* out);
+
 /* ---- functor probes (run the device functors on n inputs; host arrays) ---- */
 /* reconstruct<recon, limiter>: stencils[n][5] (s[-2..2]) -> wl[n], wr[n]; prec 0 FP64, 1 FP32. */
 int mh_probe_reconstruct(int32_t recon, int32_t limiter, int32_t prec, int64_t n, const double* stencils,

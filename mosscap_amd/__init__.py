@@ -102,6 +102,21 @@ _SIGNATURES = [
     ("mh_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     ("mh_get_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
     ("mh_stream", C.c_void_p, [C.c_void_p]),
+    ("mh_batch_create", C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.POINTER(MhBoundaries),
+                                  C.POINTER(MhSchemes), C.c_double, C.POINTER(MhOptions), C.POINTER(C.c_void_p)]),
+    ("mh_batch_destroy", None, [C.c_void_p]),
+    ("mh_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("mh_batch_num_values", C.c_int64, [C.c_void_p]),
+    ("mh_batch_offsets", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_batch_upload", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_batch_download", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_batch_fill_bcs", C.c_int, [C.c_void_p]),
+    ("mh_batch_run", C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_double, C.c_int32, C.c_int32]),
+    ("mh_batch_times", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_batch_set_waves", C.c_int, [C.c_void_p, C.c_int32]),
+    ("mh_batch_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
+    ("mh_batch_stream", C.c_void_p, [C.c_void_p]),
+    ("mh_batch_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
     ("mh_probe_reconstruct", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                        C.c_void_p]),
     ("mh_probe_riemann", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int64, C.c_void_p,
@@ -293,6 +308,99 @@ class HydroContext:
     @property
     def stream(self) -> int:
         return lib().mh_stream(self._h) or 0
+
+
+class BatchContext:
+    """An ensemble of independent 1-D problems of uneven length on the GPU (mh_batch_*)."""
+
+    def __init__(self, lengths, ng=3, x_dim=1.0, bc=("zerograd", "zerograd"), bc_const=((0.0,) * 5,) * 2,
+                 reconstruction="weno5z", slope_limiter="monotonizedcentral", riemann_solver="hllc",
+                 time_stepper="ssprk3", gamma=1.4, precision_form=FP64, rho_floor=0.0, p_floor=0.0):
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.int32)
+        self.ng = ng
+        b = MhBoundaries()
+        for n, kind, cv in zip(("xs", "xe"), bc, bc_const):
+            setattr(b, n, BOUNDARY[kind.lower()])
+            arr = getattr(b, n + "_const")
+            for i, v in enumerate(cv):
+                arr[i] = v
+        sch = MhSchemes(RECONSTRUCTION[reconstruction], SLOPE_LIMITER[slope_limiter], RIEMANN_SOLVER[riemann_solver],
+                        TIME_STEPPER[time_stepper])
+        opt = MhOptions(precision_form, 0, rho_floor, p_floor)
+        h = C.c_void_p()
+        code = lib().mh_batch_create(len(self.lengths), _ptr(self.lengths), ng, x_dim, C.byref(b), C.byref(sch),
+                                     gamma, C.byref(opt), C.byref(h))
+        if code != MH_OK:
+            raise MoscapError(code, lib().mh_batch_last_error(None).decode())
+        self._h = h
+        self.n_values = lib().mh_batch_num_values(h)
+        off = np.zeros(len(self.lengths), dtype=np.int64)
+        self._c(lib().mh_batch_offsets(h, _ptr(off)))
+        self.offsets = off
+
+    def _c(self, code):
+        if code != MH_OK:
+            raise MoscapError(code, lib().mh_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mh_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def block(self, q: np.ndarray, p: int) -> np.ndarray:
+        """View of problem p's [3][n_p + 2 ng] block inside the packed array q."""
+        xc = int(self.lengths[p]) + 2 * self.ng
+        o = int(self.offsets[p])
+        return q[o:o + 3 * xc].reshape(3, xc)
+
+    def upload(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == self.n_values
+        self._c(lib().mh_batch_upload(self._h, _ptr(q)))
+
+    def download(self):
+        q = np.empty(self.n_values)
+        self._c(lib().mh_batch_download(self._h, _ptr(q)))
+        return q
+
+    def fill_bcs(self):
+        self._c(lib().mh_batch_fill_bcs(self._h))
+
+    def run(self, nsteps, max_cfl=0.4, max_time=1e30, persistent=True, use_graph=True):
+        self._c(lib().mh_batch_run(self._h, nsteps, max_cfl, max_time, 0 if persistent else 1, 1 if use_graph else 0))
+
+    def times(self):
+        t = np.empty(len(self.lengths))
+        dt = np.empty(len(self.lengths))
+        self._c(lib().mh_batch_times(self._h, _ptr(t), _ptr(dt)))
+        return t, dt
+
+    def set_waves(self, waves: int):
+        self._c(lib().mh_batch_set_waves(self._h, waves))
+
+    def autotune(self, steps=3, waves_per_cu=None):
+        cand = np.array(list(waves_per_cu or [2, 4, 6, 8, 12, 16, 24, 32]) + [0], dtype=np.int32)
+        ms = np.zeros(len(cand))
+        best = C.c_int32()
+        self._c(lib().mh_batch_autotune(self._h, steps, _ptr(cand), C.byref(best), _ptr(ms)))
+        return best.value, {int(c): float(m) for c, m in zip(cand[:-1], ms[:-1]) if m > 0}
+
+    def stats(self) -> MhStats:
+        s = MhStats()
+        self._c(lib().mh_batch_stats(self._h, C.byref(s)))
+        return s
+
+    @property
+    def stream(self) -> int:
+        return lib().mh_batch_stream(self._h) or 0
 
 
 def run_to_time(ctx: HydroContext, time: float = 0.0, prev_output_time: float = 0.0, max_steps: int = -1):

@@ -46,7 +46,10 @@ UNITS = [
     ("kernels/stage_f64.hip", "off"),
     ("kernels/stage_f32c.hip", "fast"),
     ("kernels/common.hip", "off"),
+    ("kernels/batch_f64.hip", "off"),
+    ("kernels/batch_f32c.hip", "fast"),
     ("capi.cpp", "off"),
+    ("capi_batch.cpp", "off"),
 ]
 
 

@@ -1,0 +1,95 @@
+// Batched ensemble of independent 1-D problems of uneven length.
+//
+// Layout: problem p owns a contiguous block [3][xc_p] (xc_p = n_p + 2 ng) at
+// element offset base_p; the blocks are packed back to back. A host-built work
+// list holds one item per (problem, 62-cell segment). One persistent launch
+// per stage walks the list (wave w takes items w, w + grid, ...), so every
+// wavefront does equal work whatever the length mix; each problem keeps its
+// own dx, dt and time, and the CFL max is reduced per problem with atomics.
+#pragma once
+
+#include "ghost_dt.hpp"
+#include "stage_core.hpp"
+
+namespace mh {
+
+struct BatchArrays {
+    const int2* items;          // (problem, first output cell i0)
+    const int* base;            // element offset of each problem's block
+    const int* xc;              // cells incl. ghosts
+    const double* dx;
+    const double* dt;           // per-problem dt of the current step
+    unsigned long long* amax;   // per-problem CFL max (bit pattern)
+    double* time;               // per-problem clock
+};
+
+struct BatchGrid {
+    int32_t ng, nitems, nprob, kind, reduce_dt;
+};
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__device__ inline void batch_stage(const BatchGrid& g, const typename Form::Ptrs& p, const BatchArrays& b) {
+    using T = typename Form::real;
+    constexpr int ORD = stencil_order(R);
+    __shared__ T lds[3 * (kWave + 2 * ORD)];
+    const int lane = threadIdx.x & 63;
+    unsigned int hits = 0;
+    for (int it = blockIdx.x; it < g.nitems; it += gridDim.x) {
+        const int2 w = b.items[it];
+        const int pr = w.x, i0 = w.y;
+        const int xc = b.xc[pr];
+        const T dt = (T)b.dt[pr];
+        const T dtdx = dt / (T)b.dx[pr];
+        const T src = T(0) * dt;
+        double amax = 0.0;
+        stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, b.base[pr], xc, g.ng, xc - 2 * g.ng, i0, dtdx,
+                                         src, lds, amax, hits);
+        if (g.reduce_dt) {
+            const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
+            if (lane == 0) atomicMax(&b.amax[pr], (unsigned long long)m);
+        }
+    }
+    if (p.floor_hits != nullptr) {
+        const unsigned int total = wave_sum_u32(hits);
+        if (lane == 0 && total != 0u) atomicAdd(p.floor_hits, total);
+    }
+}
+
+// Ghost fill of every problem (Boundaries.hpp:11-119 with Axis = 0, NumDim =
+// 1): one thread per (problem, ghost cell); sources are interior cells.
+template <class Form>
+__device__ inline void batch_ghost(const GhostGrid& g, int nprob, typename Form::Planes in, typename Form::Planes out,
+                                   const BatchArrays& b) {
+    using Raw = typename Form::raw;
+    constexpr int NV = 3;
+    const int ng = g.ng;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    int pr = -1;
+    if (tid < nprob * 2 * ng) {
+        pr = tid / (2 * ng);
+        const int pidx = tid % (2 * ng);
+        const int xc = b.xc[pr], base = b.base[pr];
+        const int i = pidx < ng ? pidx : (xc - 1) - (pidx - ng);
+        const bool start = i < ng;
+        const BoundaryType bt = (BoundaryType)(start ? g.bc[0] : g.bc[1]);
+        Raw q[NV];
+        if (bt == BoundaryType::UserFn) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) q[k] = Form::get(in, base + k * xc + i);
+        } else {
+            if (bt != BoundaryType::Constant) {
+                const int is = ghost_source(i, xc, ng, bt);
+#pragma unroll
+                for (int k = 0; k < NV; ++k) q[k] = Form::get(out, base + k * xc + is);
+            }
+            ghost_modify<Form, 1, NV, 0>(q, bt, start, start ? g.bconst[0] : g.bconst[1]);
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) Form::put(out, base + k * xc + i, q[k]);
+        if (g.reduce_dt) a = Form::template signal_speed_nd<1>(g.gamma, q);
+    }
+    if (g.reduce_dt && pr >= 0 && a > 0.0) atomicMax(&b.amax[pr], (unsigned long long)__builtin_bit_cast(uint64_t, a));
+}
+
+}  // namespace mh

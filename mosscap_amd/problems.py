@@ -129,3 +129,37 @@ def sedov_config(n: int = 4096) -> Config:
     return Config(num_dim=2, nx=n, ny=n, ng=3, x_dim=2.0, x_start=-1.0, y_start=-1.0, bc=("zerograd",) * 4,
                   reconstruction="ppm", slope_limiter="monotonizedcentral", riemann_solver="hll",
                   time_stepper="ssprk3", gamma=1.4, max_cfl=0.3, max_time=0.05, output_delta=0.1)
+
+
+def riemann_batch(nprob: int = 4096, seed: int = 4096, n_min: int = 128, n_max: int = 4096, ng: int = 3,
+                  gamma: float = 1.4):
+    """Ensemble of 1-D Riemann problems on [0, 1]: lengths log-uniform in [n_min, n_max], interface at
+    x = 0.5; left/right rho log-uniform in [0.1, 10], p log-uniform in [0.01, 100], vx uniform in
+    [-1, 1], drawn per problem from the counter stream of `seed`. Returns (lengths, packed Q, states)
+    with Q the back-to-back [3][n + 2 ng] blocks, every cell (ghosts included) set."""
+    k = np.arange(nprob, dtype=np.uint64) * np.uint64(8)
+    u = [uniform01(seed, k + np.uint64(j)) for j in range(7)]
+    lengths = np.floor(np.exp(np.log(n_min) + u[0] * (np.log(n_max + 1) - np.log(n_min)))).astype(np.int64)
+    lengths = np.clip(lengths, n_min, n_max)
+    rho_l = 10.0 ** (-1.0 + 2.0 * u[1])
+    rho_r = 10.0 ** (-1.0 + 2.0 * u[2])
+    p_l = 10.0 ** (-2.0 + 4.0 * u[3])
+    p_r = 10.0 ** (-2.0 + 4.0 * u[4])
+    v_l = 2.0 * u[5] - 1.0
+    v_r = 2.0 * u[6] - 1.0
+    blocks = []
+    for i in range(nprob):
+        n = int(lengths[i])
+        dx = 1.0 / n
+        x = np.arange(n + 2 * ng) * dx + (-(ng - 0.5) * dx)
+        left = x < 0.5
+        rho = np.where(left, rho_l[i], rho_r[i])
+        v = np.where(left, v_l[i], v_r[i])
+        p = np.where(left, p_l[i], p_r[i])
+        q = np.empty((3, n + 2 * ng))
+        q[0] = rho
+        q[1] = rho * v
+        q[2] = p / (gamma - 1.0) + 0.5 * rho * (v * v)
+        blocks.append(q.ravel())
+    states = np.stack([rho_l, v_l, p_l, rho_r, v_r, p_r], axis=1)
+    return lengths.astype(np.int32), np.concatenate(blocks), states

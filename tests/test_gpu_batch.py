@@ -1,0 +1,82 @@
+"""Batched ensemble of uneven 1-D problems (one persistent launch per stage):
+every problem against its own oracle run, bitwise in FP64; the persistent and
+one-launch-per-problem modes, graphs on and off, give identical states."""
+import numpy as np
+import pytest
+
+import mosscap_amd as M
+from mosscap_amd import BatchContext, Config, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(nprob=24, seed=7, n_max=400):
+    lengths, q0, _ = problems.riemann_batch(nprob=nprob, seed=seed, n_min=40, n_max=n_max)
+    return lengths, q0
+
+
+@pytest.mark.parametrize("recon,solver,stepper", [("weno5z", "hllc", "ssprk3"), ("muscl", "hll", "rk2"),
+                                                  ("ppm", "rusanov", "ssprk4")])
+def test_batch_fp64_matches_oracle_per_problem(orc, recon, solver, stepper):
+    lengths, q0 = _small()
+    nsteps = 15
+    kw = dict(reconstruction=recon, slope_limiter="minmod", riemann_solver=solver, time_stepper=stepper)
+    with BatchContext(lengths, **kw) as b:
+        b.upload(q0)
+        b.fill_bcs()
+        b.run(nsteps, max_cfl=0.4)
+        q = b.download()
+        times, _ = b.times()
+        for p, n in enumerate(lengths):
+            cfg = Config(num_dim=1, nx=int(n), ng=3, bc=("zerograd", "zerograd", "wall", "wall"), max_cfl=0.4,
+                         max_time=1e30, output_delta=1e30, **kw)
+            sim = orc.OracleSim(cfg)
+            sim.set_q(b.block(q0, p).reshape(cfg.shape))
+            sim.fill_bcs()
+            t, _, _ = sim.run(max_steps=nsteps)
+            assert times[p] == t, p
+            np.testing.assert_array_equal(b.block(q, p), sim.get_q().reshape(3, -1), err_msg=f"problem {p}")
+
+
+def test_batch_modes_identical():
+    lengths, q0 = _small(nprob=40, seed=3, n_max=900)
+    out = []
+    for persistent, graph in [(True, True), (True, False), (False, False), (False, True)]:
+        with BatchContext(lengths) as b:
+            b.upload(q0)
+            b.fill_bcs()
+            if persistent:
+                b.set_waves(7)  # a grid much smaller than the work list
+            b.run(6, persistent=persistent, use_graph=graph)
+            out.append((b.download(), b.times()[0]))
+    for q, t in out[1:]:
+        np.testing.assert_array_equal(q, out[0][0])
+        np.testing.assert_array_equal(t, out[0][1])
+
+
+def test_batch_fp32c_close_to_fp64():
+    lengths, q0 = _small(nprob=64, seed=11, n_max=2000)
+    res = []
+    for form in (M.FP64, M.FP32C):
+        with BatchContext(lengths, precision_form=form) as b:
+            b.upload(q0)
+            b.fill_bcs()
+            b.run(100)
+            res.append(b.download())
+    a, r = res[1], res[0]
+    rel = np.abs(a - r).sum() / np.abs(r).sum()
+    assert rel < 1e-5, rel
+
+
+def test_batch_autotune_restores_state():
+    lengths, q0 = _small(nprob=32, seed=5, n_max=600)
+    with BatchContext(lengths) as a, BatchContext(lengths) as b:
+        a.upload(q0)
+        a.fill_bcs()
+        b.upload(q0)
+        b.fill_bcs()
+        best, table = a.autotune(steps=2, waves_per_cu=[1, 4])
+        assert len(table) == 2
+        a.run(5)
+        b.run(5)
+        np.testing.assert_array_equal(a.download(), b.download())
