@@ -180,6 +180,88 @@ def measured_traffic(kernel_prefix):
     return sum(v["traffic"] for v in vals) / len(vals)
 
 
+def bench_batch(a):
+    """Ensemble of 4096 uneven 1-D Riemann problems (WENO5-Z + HLLC + SSPRK3, CFL 0.4,
+    FP32C): one persistent launch per stage (autotuned grid) against one launch per problem."""
+    import mosscap_amd as M
+    from mosscap_amd import problems
+    lengths, q0, _ = problems.riemann_batch(nprob=4096)
+    cells = int(lengths.sum())
+    out = {"metric": "Mcell-updates/s (batched 1-D ensemble)", "unit": "Mcell-updates/s", "n_gpus": 1,
+           "dtype": "f32c", "data": "synthetic (seeded ensemble)",
+           "config": {"workload": f"4096 1-D Riemann problems, lengths log-uniform in [128, 4096] "
+                                  f"({cells} cells), WENO5-Z + HLLC + SSPRK3, CFL 0.4, per-problem dt"}}
+    # positivity floors on: the plain method goes non-finite on near-vacuum members
+    with M.BatchContext(lengths, precision_form=M.FP32C, rho_floor=1e-10, p_floor=1e-12) as b:
+        b.upload(q0)
+        b.fill_bcs()
+        best, table = b.autotune(steps=5)
+        out["autotune_waves"] = {"best": best, "ms_per_step_by_waves_per_cu": table}
+        b.upload(q0)
+        b.fill_bcs()
+        b.run(a.warmup)
+        _sync()
+        t0 = time.perf_counter()
+        b.run(a.batch_steps)
+        _sync()
+        el = time.perf_counter() - t0
+        out["steps"] = a.batch_steps
+        out["value"] = round(cells * a.batch_steps / el / 1e6, 2)
+        out["ms_per_step"] = round(el * 1e3 / a.batch_steps, 4)
+        out["floor_hits"] = int(b.stats().floor_hits)
+        # one launch per problem and stage (graph-captured too), fewer steps
+        n2 = max(2, a.batch_steps // 25)
+        b.upload(q0)
+        b.fill_bcs()
+        b.run(1, persistent=False, use_graph=True)
+        _sync()
+        t0 = time.perf_counter()
+        b.run(n2, persistent=False, use_graph=True)
+        _sync()
+        el2 = time.perf_counter() - t0
+        out["per_problem_launches"] = {"steps": n2, "value": round(cells * n2 / el2 / 1e6, 2),
+                                       "ms_per_step": round(el2 * 1e3 / n2, 3)}
+        out["persistent_over_per_problem"] = round(out["value"] / out["per_problem_launches"]["value"], 2)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_sedov(a):
+    """Sedov blast (PPM-MC + HLL + SSPRK3, CFL 0.3, outflow) to t = 0.05: FP32C with FP64 pressure
+    recovery, compensated state and positivity floors against all-FP64."""
+    import mosscap_amd as M
+    from mosscap_amd import problems
+    cfg = problems.sedov_config(a.n)
+    q0 = problems.sedov(cfg)
+    res = {}
+    for form in (M.FP64, M.FP32C):
+        kw = dict(rho_floor=1e-10, p_floor=1e-12) if form == M.FP32C else {}
+        with M.HydroContext(cfg, precision_form=form, **kw) as ctx:
+            ctx.upload(q0)
+            ctx.fill_bcs()
+            _sync()
+            t0 = time.perf_counter()
+            t, p, steps = 0.0, 0.0, 0
+            while t < cfg.max_time:
+                t, p = ctx.run(50, time=t, prev_output_time=p)
+                steps += 50
+            _sync()
+            el = time.perf_counter() - t0
+            st = ctx.stats()
+            res[form] = (ctx.download(), el, int(st.steps), t, int(st.floor_hits))
+    q64, el64, s64, t64, _ = res[M.FP64]
+    q32, el32, s32, t32, hits = res[M.FP32C]
+    cells = cfg.nx * cfg.ny
+    out = {"metric": "Sedov FP32C vs FP64", "unit": "Mcell-updates/s", "n_gpus": 1,
+           "config": {"workload": f"Sedov {cfg.nx}x{cfg.ny} on [-1,1]^2, PPM(MC) + HLL + SSPRK3, CFL 0.3, "
+                                  "zerograd, t_end 0.05 (steps past t_end have dt = 0)"},
+           "fp64": {"steps": s64, "t": t64, "value": round(cells * s64 / el64 / 1e6, 1)},
+           "fp32c": {"steps": s32, "t": t32, "value": round(cells * s32 / el32 / 1e6, 1), "floor_hits": hits},
+           "rel_l1_fp32c_vs_fp64": [float(f"{r:.3e}") for r in _rel_l1(q32, q64, cfg.ng)]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,7 +279,16 @@ def main(argv=None):
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="summarise rocprofv3 --pmc CSVs into the traffic JSON (CPU only) and exit")
     ap.add_argument("--copy-bytes", type=int, default=512 << 20)
+    ap.add_argument("--workload", choices=["kh", "batch", "sedov"], default="kh",
+                    help="kh: the headline line; batch / sedov: the other BASELINE workloads")
+    ap.add_argument("--batch-steps", type=int, default=500)
     a = ap.parse_args(argv)
+    if a.workload == "batch":
+        return bench_batch(a)
+    if a.workload == "sedov":
+        if a.n == 2048:
+            a.n = 4096
+        return bench_sedov(a)
     if a.pmc_summary:
         return pmc_summary(*a.pmc_summary, a.copy_bytes, TRAFFIC_JSON)
     if a.copy_probe_mb:

@@ -152,7 +152,7 @@ struct mh_ctx {
     hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce) {
         hipError_t e;
         if (fp32c()) {
-            FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, (float)gamma, gamma - 1.0,
+            FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, gamma, gamma - 1.0,
                              &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};

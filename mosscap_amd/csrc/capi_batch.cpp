@@ -91,7 +91,7 @@ struct mh_batch {
             const BatchGrid g{ng, nitems, nprob, kind, reduce};
             const int grid = std::min(waves > 0 ? waves : cus * 12, nitems);
             if (fp32c()) {
-                FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), d_dt, 0.0, (float)gamma, gamma - 1.0, d_amax,
+                FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), d_dt, 0.0, gamma, gamma - 1.0, d_amax,
                                  floor_hits, opt.rho_floor, opt.p_floor};
                 e = launch_batch_stage_f32c(sch, g, grid, p, b, stream);
                 if (e == hipSuccess) e = launch_batch_ghost_f32c(ghost_grid(reduce), nprob, p32(in), p32(out), b, stream);
@@ -108,7 +108,7 @@ struct mh_batch {
             const size_t o = (size_t)base[pr];
             if (fp32c()) {
                 auto P = [&](int k) { return FormF32c::Planes{qhi[k] + o, qlo[k] + o}; };
-                FormF32c::Ptrs p{P(in), P(qn_k), P(out), d_dt + pr, dx[pr], (float)gamma, gamma - 1.0, d_amax + pr,
+                FormF32c::Ptrs p{P(in), P(qn_k), P(out), d_dt + pr, dx[pr], gamma, gamma - 1.0, d_amax + pr,
                                  floor_hits, opt.rho_floor, opt.p_floor};
                 e = launch_stage1d_f32c(sch, kind, reduce, xc[pr], ng, p, stream);
                 if (e == hipSuccess) e = launch_ghost_f32c(1, gg, P(in), P(out), d_amax + pr, stream);

@@ -40,9 +40,13 @@ __device__ inline void weno5z_f32(const S& s, float& wL, float& wR) {
     const float be2 = __builtin_fmaf(k * b2, b2, a2 * a2);
     constexpr float eps = 1e-15f * (12.0f / 13.0f);
     const float tau = __builtin_fabsf(be0 - be2);
-    const float t0 = tau * rcp32(be0 + eps);
-    const float t1 = tau * rcp32(be1 + eps);
-    const float t2 = tau * rcp32(be2 + eps);
+    // tau/(beta+eps) reaches ~1e19 next to a flat stencil; clamped so that
+    // its square (and the weighted sums) stay finite in FP32. A weight ratio
+    // of 1e30 keeps the ENO choice exact to FP32 precision.
+    constexpr float tmax = 1e15f;
+    const float t0 = __builtin_fminf(tau * rcp32(be0 + eps), tmax);
+    const float t1 = __builtin_fminf(tau * rcp32(be1 + eps), tmax);
+    const float t2 = __builtin_fminf(tau * rcp32(be2 + eps), tmax);
     const float q0 = __builtin_fmaf(t0, t0, 1.0f);
     const float q1 = __builtin_fmaf(t1, t1, 1.0f);
     const float q2 = __builtin_fmaf(t2, t2, 1.0f);
@@ -59,25 +63,80 @@ __device__ inline void weno5z_f32(const S& s, float& wL, float& wR) {
     wR = __builtin_fmaf(nR, rcp32(6.0f * (A2 + A1 + q0)), s0);
 }
 
+// EOS constants of the FP32 functors, rounded once from FP64 on the host so
+// that E = p/(gamma-1) + ... in the fluxes inverts the FP64 pressure recovery.
+struct Eos32 {
+    float gamma;  // float(gamma)
+    float igm1;   // float(1 / (gamma - 1))
+    float gfac;   // float((gamma + 1) / (2 gamma))
+};
+
+template <int NumDim, typename W>
+__device__ inline float energy_f32(const Eos32& e, const W& w) {
+    float v2 = w[1] * w[1];
+#pragma unroll
+    for (int a = 2; a <= NumDim; ++a) v2 = __builtin_fmaf(w[a], w[a], v2);
+    return __builtin_fmaf(w[1 + NumDim], e.igm1, 0.5f * w[0] * v2);
+}
+
+// Rusanov (Riemann.hpp:22-46) and HLL (48-73, with its sR quirk) in FP32.
+template <bool HLL, int Axis, int NumDim, typename WL, typename WR, typename FT>
+__device__ inline void rusanov_hll_f32(const Eos32& e, const WL& wL, const WR& wR, FT& flux) {
+    constexpr int IV = 1 + Axis;
+    constexpr int IE = 1 + NumDim;
+    const float gpL = e.gamma * wL[IE], gpR = e.gamma * wR[IE];
+    const float csL = gpL * rsq32(gpL * wL[0]);
+    const float csR = gpR * rsq32(gpR * wR[0]);
+    const float vL = wL[IV], vR = wR[IV];
+    const float EL = energy_f32<NumDim>(e, wL), ER = energy_f32<NumDim>(e, wR);
+    float qL[2 + NumDim], qR[2 + NumDim], fL[2 + NumDim], fR[2 + NumDim];
+    qL[0] = wL[0];
+    qR[0] = wR[0];
+    fL[0] = wL[0] * vL;
+    fR[0] = wR[0] * vR;
+#pragma unroll
+    for (int a = 1; a <= NumDim; ++a) {
+        qL[a] = wL[0] * wL[a];
+        qR[a] = wR[0] * wR[a];
+        fL[a] = fL[0] * wL[a];
+        fR[a] = fR[0] * wR[a];
+    }
+    fL[IV] += wL[IE];
+    fR[IV] += wR[IE];
+    qL[IE] = EL;
+    qR[IE] = ER;
+    fL[IE] = (EL + wL[IE]) * vL;
+    fR[IE] = (ER + wR[IE]) * vR;
+    if constexpr (HLL) {
+        constexpr float tiny = 1e-20f;
+        const float a = __builtin_fminf(vL - csL, vR - csR);
+        const float sL = (a < -tiny) ? a : -tiny;
+        const float b = __builtin_fmaxf(vL + csL, vL + csR);  // the reference's vL twice
+        const float sR = (-tiny < b) ? b : -tiny;
+        const float sM = rcp32(sR - sL);
+        const float srl = sR * sL;
+#pragma unroll
+        for (int i = 0; i < 2 + NumDim; ++i)
+            flux[i] = sM * __builtin_fmaf(srl, qR[i] - qL[i], __builtin_fmaf(sR, fL[i], -sL * fR[i]));
+    } else {
+        const float max_c = 0.5f * ((csL + __builtin_fabsf(vL)) + (csR + __builtin_fabsf(vR)));
+#pragma unroll
+        for (int i = 0; i < 2 + NumDim; ++i) flux[i] = 0.5f * __builtin_fmaf(-max_c, qR[i] - qL[i], fL[i] + fR[i]);
+    }
+}
+
 template <int Axis, int NumDim, typename WL, typename WR, typename FT>
-__device__ inline void hllc_f32(const float gamma, const WL& wL, const WR& wR, FT& flux) {
+__device__ inline void hllc_f32(const Eos32& e, const WL& wL, const WR& wR, FT& flux) {
     constexpr int IV = 1 + Axis;
     constexpr int IE = 1 + NumDim;
     constexpr float tiny = 1e-20f;
-    const float igm1 = rcp32(gamma - 1.0f);         // hoisted out of loops
-    const float gfac = (gamma + 1.0f) * rcp32(2.0f * gamma);
+    const float gamma = e.gamma, gfac = e.gfac;
     const float rL = wL[0], rR = wR[0], pL = wL[IE], pR = wR[IE], vL = wL[IV], vR = wR[IV];
     const float gpL = gamma * pL, gpR = gamma * pR;
     const float csL = gpL * rsq32(gpL * rL);
     const float csR = gpR * rsq32(gpR * rR);
-    float v2L = wL[1] * wL[1], v2R = wR[1] * wR[1];
-#pragma unroll
-    for (int a = 2; a <= NumDim; ++a) {
-        v2L = __builtin_fmaf(wL[a], wL[a], v2L);
-        v2R = __builtin_fmaf(wR[a], wR[a], v2R);
-    }
-    const float EL = __builtin_fmaf(pL, igm1, 0.5f * rL * v2L);
-    const float ER = __builtin_fmaf(pR, igm1, 0.5f * rR * v2R);
+    const float EL = energy_f32<NumDim>(e, wL);
+    const float ER = energy_f32<NumDim>(e, wR);
     const float rho_avg = 0.5f * (rL + rR);
     const float cs_avg = 0.5f * (csL + csR);
     const float p_mid = 0.5f * __builtin_fmaf((vL - vR) * rho_avg, cs_avg, pL + pR);

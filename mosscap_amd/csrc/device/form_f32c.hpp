@@ -78,7 +78,7 @@ struct FormF32c {
         Planes qs, qn, out;
         const double* dt;
         double dx;
-        float gamma;
+        double gamma;
         double gm1;  // gamma - 1 in FP64, for the pressure recovery
         unsigned long long* amax_bits;
         unsigned int* floor_hits;
@@ -105,15 +105,22 @@ struct FormF32c {
     }
 
     // cons->prim with FP64 pressure recovery. Returns true when a floor fired.
+    // The density floor applies before 1/rho is formed, so a state whose
+    // density went non-positive still yields finite velocities.
     template <int ND, int NV>
     __device__ static inline bool to_prim_nd(const Ptrs& p, const float2 (&q)[NV], float (&w)[NV]) {
-        const double rho_d = df::to_double(q[0]);
+        double rho_d = df::to_double(q[0]);
+        bool hit = false;
+        const bool floors = p.rho_floor > 0.0f;
+        if (floors && rho_d < (double)p.rho_floor) {
+            rho_d = (double)p.rho_floor;
+            hit = true;
+        }
         // 1/rho: hardware FP32 reciprocal refined by one FP64 Newton step
         // (relative error ~1e-14, ample for the kinetic-energy subtraction).
         const double r0 = (double)__builtin_amdgcn_rcpf((float)rho_d);
         const double r = r0 * (2.0 - rho_d * r0);
         double m2 = 0.0;
-        float rho = (float)rho_d;
 #pragma unroll
         for (int a = 0; a < ND; ++a) {
             const double m = df::to_double(q[1 + a]);
@@ -121,18 +128,11 @@ struct FormF32c {
             w[1 + a] = (float)(m * r);
         }
         double pres = p.gm1 * (df::to_double(q[1 + ND]) - 0.5 * m2 * r);
-        bool hit = false;
-        if (p.rho_floor > 0.0f) {
-            if (rho < p.rho_floor) {
-                rho = p.rho_floor;
-                hit = true;
-            }
-            if (pres < (double)p.p_floor) {
-                pres = (double)p.p_floor;
-                hit = true;
-            }
+        if (floors && pres < (double)p.p_floor) {
+            pres = (double)p.p_floor;
+            hit = true;
         }
-        w[0] = rho;
+        w[0] = (float)rho_d;
         w[1 + ND] = (float)pres;
         return hit;
     }
@@ -157,12 +157,26 @@ struct FormF32c {
         }
     }
 
+    // Reconstructed face states get the same floors as the cell states
+    // (WENO and PPM may undershoot between positive cell values).
+    template <int ND, int NV>
+    __device__ static inline void floor_face(const Ptrs& p, float (&w)[NV]) {
+        if (p.rho_floor > 0.0f) {
+            w[0] = w[0] < p.rho_floor ? p.rho_floor : w[0];
+            w[1 + ND] = w[1 + ND] < p.p_floor ? p.p_floor : w[1 + ND];
+        }
+    }
+
+    // CFL signal speed in FP64; a state whose pressure went negative
+    // contributes its sound speed at p = 0 (the floors keep the scheme going).
     template <int ND, int NV>
     __device__ static inline double signal_speed_nd(double gamma, const float2 (&q)[NV]) {
         double qd[NV], w[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) qd[v] = df::to_double(q[v]);
+        qd[0] = qd[0] > 1e-300 ? qd[0] : 1e-300;
         cons_to_prim<ND>(gamma, qd, w);
+        w[1 + ND] = w[1 + ND] > 0.0 ? w[1 + ND] : 0.0;
         return mh::signal_speed<ND>(gamma, w);
     }
 };

@@ -44,6 +44,10 @@ struct FormF64 {
         return false;  // no floors in the drop-in
     }
 
+    // no floors in the drop-in (the reference has none)
+    template <int ND, int NV>
+    __device__ static inline void floor_face(const Ptrs&, double (&)[NV]) {}
+
     // TimeStepping.cpp stage lambdas (RK2 60-67/80-87, SSPRK3 137-144/157-164/
     // 177-184, SSPRK(4,3) 234-301), grouped exactly as written there.
     __device__ static inline double combine(int kind, double Q, double Qn, double dq, double src) {

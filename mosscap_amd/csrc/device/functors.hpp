@@ -65,6 +65,25 @@ struct EosT {
     T gamma;
 };
 
+// The FP32 functors take the EOS constants rounded once from FP64 (fast32.hpp).
+template <>
+struct EosT<float> : Eos32 {};
+
+template <typename T>
+__host__ __device__ inline EosT<T> make_eos(double gamma);
+template <>
+__host__ __device__ inline EosT<double> make_eos<double>(double gamma) {
+    return EosT<double>{gamma};
+}
+template <>
+__host__ __device__ inline EosT<float> make_eos<float>(double gamma) {
+    EosT<float> e;
+    e.gamma = (float)gamma;
+    e.igm1 = (float)(1.0 / (gamma - 1.0));
+    e.gfac = (float)((gamma + 1.0) / (2.0 * gamma));
+    return e;
+}
+
 // sound_speed, Eos.hpp:54-59
 template <int NumDim, typename T, typename WType>
 __host__ __device__ inline T sound_speed(const T gamma, const WType& w) {
@@ -256,7 +275,9 @@ __host__ __device__ inline void riemann_flux(const EosT<T>& eos, const WL& wL, c
     const T gamma = eos.gamma;
     T qL[n_hydro], qR[n_hydro], fL[n_hydro], fR[n_hydro];
 
-    if constexpr (rs == RiemannSolver::Rusanov) {  // 22-46
+    if constexpr (std::is_same_v<T, float> && rs != RiemannSolver::Hllc) {  // precision form
+        rusanov_hll_f32<rs == RiemannSolver::Hll, Axis, NumDim>(eos, wL, wR, flux);
+    } else if constexpr (rs == RiemannSolver::Rusanov) {  // 22-46
         const T vL = wL[IV];
         const T vR = wR[IV];
         prim_to_cons<NumDim>(gamma, wL, qL);
@@ -288,7 +309,7 @@ __host__ __device__ inline void riemann_flux(const EosT<T>& eos, const WL& wL, c
             flux[i] = sM * (sR * fL[i] - sL * fR[i] + sR * sL * (qR[i] - qL[i]));
         }
     } else if constexpr (std::is_same_v<T, float>) {  // precision form
-        hllc_f32<Axis, NumDim>(gamma, wL, wR, flux);
+        hllc_f32<Axis, NumDim>(eos, wL, wR, flux);
     } else {  // HLLC, 75-176 (Athena-style pressure-based wave speeds)
         const T csL = sound_speed<NumDim>(gamma, wL);
         const T csR = sound_speed<NumDim>(gamma, wR);

@@ -84,8 +84,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int h_entry = lane < ORD ? lane : kWave + lane;
     const int h_col = min(i0 - 1 - ORD + h_entry, g.xc - 1);
 
-    const T gamma = p.gamma;
-    const EosT<T> eos{gamma};
+    const EosT<T> eos = make_eos<T>((double)p.gamma);
     const T dt = (T)p.dt[0];
     const T dtdx = dt / (T)p.dx;
     const T src = T(0) * dt;
@@ -130,6 +129,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             auto s = [&](int o) -> T { return win[ORD + o][v]; };
             reconstruct_stencil<R, SL, T>(s, wl[v], wr[v]);
         }
+        Form::template floor_face<2>(p, wl);
+        Form::template floor_face<2>(p, wr);
     };
 
     // ---- prologue: rows j0-1-ORD .. j0+ORD; y faces up to j0 ------------------
@@ -189,6 +190,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             auto s = [&](int o) -> T { return ring[slot][v][lane + ORD + o]; };
             reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
         }
+        Form::template floor_face<2>(p, xl);
+        Form::template floor_face<2>(p, xr);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
         riemann_flux<RS, 0, 2>(eos, xrm, xl, Fx);
@@ -244,7 +247,7 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
     const bool halo_lane = lane < 2 * ORD;
     const int h_entry = lane < ORD ? lane : kWave + lane;
     const int h_col = min(i0 - 1 - ORD + h_entry, xc - 1);
-    const EosT<T> eos{p.gamma};
+    const EosT<T> eos = make_eos<T>((double)p.gamma);
 
     Raw q[NV], qn[NV], qh[NV];
 #pragma unroll
@@ -272,6 +275,8 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
         auto s = [&](int o) -> T { return lds_row[v * ROW + lane + ORD + o]; };
         reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
     }
+    Form::template floor_face<1>(p, xl);
+    Form::template floor_face<1>(p, xr);
 #pragma unroll
     for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
     riemann_flux<RS, 0, 1>(eos, xrm, xl, Fx);

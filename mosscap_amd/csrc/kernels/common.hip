@@ -80,7 +80,7 @@ __global__ void probe_riemann_kernel(double gamma, int n, const double* wl, cons
         a[k] = (T)wl[NH * i + k];
         b[k] = (T)wr[NH * i + k];
     }
-    riemann_flux<RS, Axis, ND>(EosT<T>{(T)gamma}, a, b, f);
+    riemann_flux<RS, Axis, ND>(make_eos<T>(gamma), a, b, f);
     for (int k = 0; k < NH; ++k) flux[NH * i + k] = (double)f[k];
 }
 
