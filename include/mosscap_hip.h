@@ -151,6 +151,30 @@ int mh_get_stats(mh_ctx* ctx, mh_stats* out);
 /* HIP stream the context launches on (a hipStream_t), for event timing. */
 void* mh_stream(mh_ctx* ctx);
 
+/* ---- discrete adjoint (no counterpart in the reference) ----------------------
+ * FP64 2-D contexts; every boundary type but symmetricoutflowdiode. Time step
+ * sizes are frozen: the backward pass differentiates the map
+ * Q^0 -> Q^N of the recorded steps with their dt held fixed.
+ *
+ * mh_adjoint_forward runs nsteps steps from the current state, keeping a
+ * checkpoint every `checkpoint_every` steps. With dts_in, those step sizes are
+ * used; otherwise each dt is the CFL dt of its state (Hydro.cpp:208-229,
+ * no output-time clamp). The sizes used are written to dts_out (nsteps
+ * values, may be NULL).
+ *
+ * mh_adjoint_backward takes dJ/dQ^N (lambda_final, host array laid out like
+ * Q, ghost entries ignored) and returns dJ/dQ^0 (lambda_initial, host, ghost
+ * entries zero), recomputing each checkpoint segment's stage states and
+ * running the reverse stages (hand-written VJPs of cons->prim, WENO5-Z and
+ * Rusanov; forward-mode dual numbers for the other functors). */
+int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double max_cfl, const double* dts_in,
+                       double* dts_out);
+int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_initial);
+/* probe of the hand-written VJPs against dual-number Jacobians (what: 0/1 WENO5-Z
+ * hand/dual, in[n][5], cot[n][2] -> out[n][5]; 2/3 Rusanov x hand/dual, 4/5 Rusanov y,
+ * in[n][8] (wL, wR), cot[n][4] -> out[n][8]). */
+int mh_probe_vjp(int32_t what, int64_t n, const double* in, const double* cot, double* out, double gamma);
+
 /* ---- batched ensemble of independent 1-D problems -------------------------
  * No counterpart in the reference (one Simulation per process there). Each
  * problem is a 1-D grid of lengths[p] interior cells on [0, x_dim] with ng

@@ -102,6 +102,9 @@ _SIGNATURES = [
     ("mh_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     ("mh_get_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
     ("mh_stream", C.c_void_p, [C.c_void_p]),
+    ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
+    ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_probe_vjp", C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     ("mh_batch_create", C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.POINTER(MhBoundaries),
                                   C.POINTER(MhSchemes), C.c_double, C.POINTER(MhOptions), C.POINTER(C.c_void_p)]),
     ("mh_batch_destroy", None, [C.c_void_p]),
@@ -305,6 +308,22 @@ class HydroContext:
         self._c(lib().mh_get_stats(self._h, C.byref(s)))
         return s
 
+    def adjoint_forward(self, nsteps: int, checkpoint_every: int = 20, dts=None, max_cfl: float | None = None):
+        """Forward steps recording dt (frozen for the backward pass) and checkpoints; returns the dts."""
+        out = np.empty(nsteps)
+        din = None if dts is None else np.ascontiguousarray(dts, dtype=np.float64)
+        self._c(lib().mh_adjoint_forward(self._h, nsteps, checkpoint_every,
+                                         self.cfg.max_cfl if max_cfl is None else max_cfl,
+                                         None if din is None else _ptr(din), _ptr(out)))
+        return out
+
+    def adjoint_backward(self, lambda_final: np.ndarray) -> np.ndarray:
+        """dJ/dQ^0 from dJ/dQ^N (arrays shaped like Q; ghost entries ignored / zero)."""
+        lf = np.ascontiguousarray(lambda_final, dtype=np.float64).reshape(self.cfg.shape)
+        out = np.empty(self.cfg.shape)
+        self._c(lib().mh_adjoint_backward(self._h, _ptr(lf), _ptr(out)))
+        return out
+
     @property
     def stream(self) -> int:
         return lib().mh_stream(self._h) or 0
@@ -437,6 +456,15 @@ def probe_riemann(solver: str, axis: int, nd: int, gamma: float, wl: np.ndarray,
     _check(lib().mh_probe_riemann(RIEMANN_SOLVER[solver], axis, nd, prec, gamma, a.shape[0], _ptr(a), _ptr(b),
                                   _ptr(f)))
     return f
+
+
+def probe_vjp(what: int, inputs: np.ndarray, cot: np.ndarray, gamma: float = 1.4) -> np.ndarray:
+    nin = 5 if what < 2 else 8
+    a = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, nin)
+    c = np.ascontiguousarray(cot, dtype=np.float64)
+    out = np.empty_like(a)
+    _check(lib().mh_probe_vjp(what, a.shape[0], _ptr(a), _ptr(c), _ptr(out), gamma))
+    return out
 
 
 def probe_math(op: int, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:

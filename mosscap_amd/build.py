@@ -48,8 +48,10 @@ UNITS = [
     ("kernels/common.hip", "off"),
     ("kernels/batch_f64.hip", "off"),
     ("kernels/batch_f32c.hip", "fast"),
+    ("kernels/adjoint.hip", "off"),
     ("capi.cpp", "off"),
     ("capi_batch.cpp", "off"),
+    ("capi_adjoint.cpp", "off"),
 ]
 
 

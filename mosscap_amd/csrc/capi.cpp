@@ -5,208 +5,13 @@
 // writes the buffer it reads (a fused stage reads its neighbours' halo
 // cells) and the reference's Q_old copy (TimeStepping.cpp:52, 129, 226)
 // disappears: Q^n stays in its buffer for the whole step.
-#include "../../include/mosscap_hip.h"
-
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "launch.hpp"
-
-using namespace mh;
+#include "ctx.hpp"
 
 namespace {
 
 thread_local std::string g_create_error;
 
-int stage_kind(TimeStepScheme ts, int stage) {
-    switch (ts) {
-        case TimeStepScheme::Rk2: return stage == 0 ? kAddFull : kRk2Final;
-        case TimeStepScheme::SspRk3: return stage == 0 ? kAddFull : stage == 1 ? kSsp3Second : kSsp3Third;
-        default: return stage == 2 ? kSsp4Third : kAddHalf;
-    }
-}
-
-bool valid_bc(int32_t b) { return b >= 0 && b <= 6; }
-
 }  // namespace
-
-struct mh_ctx {
-    int nd = 0, nvar = 0;
-    mh_grid_size sz{};
-    double dx = 0.0;
-    mh_grid_loc loc{};
-    mh_boundaries bdry{};
-    Schemes sch{};
-    double gamma = 1.4;
-    mh_options opt{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int64_t ncells = 0, nvals = 0;
-    double* q64[3] = {nullptr, nullptr, nullptr};
-    float* qhi[3] = {nullptr, nullptr, nullptr};
-    float* qlo[3] = {nullptr, nullptr, nullptr};
-    double* staging = nullptr;  // FP64 staging for the compensated form
-    Clock* clock = nullptr;
-    unsigned int* floor_hits = nullptr;
-    int cur = 0;         // buffer holding the current state
-    int qn = 0;          // buffer holding Q^n during a step
-    int next_stage = 0;  // mh_stage bookkeeping
-    int span_h = 16;
-    int strips = 0;
-    std::string err;
-    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-    // event log for mh_run_timed: (kind, start, stop); kind 0 stage, 1 ghost, 2 clock
-    struct Ev {
-        int kind;
-        hipEvent_t a, b;
-    };
-    std::vector<Ev>* evlog = nullptr;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_next = 0;
-
-    hipError_t ev_get(hipEvent_t& e) {
-        if (ev_next < ev_pool.size()) {
-            e = ev_pool[ev_next++];
-            return hipSuccess;
-        }
-        hipError_t r = hipEventCreate(&e);
-        if (r == hipSuccess) {
-            ev_pool.push_back(e);
-            ev_next = ev_pool.size();
-        }
-        return r;
-    }
-    // Runs `launch` between two recorded events when timing is on.
-    template <class F>
-    hipError_t timed(int kind, F&& launch) {
-        if (!evlog) return launch();
-        Ev ev{kind, nullptr, nullptr};
-        hipError_t e = ev_get(ev.a);
-        if (e == hipSuccess) e = ev_get(ev.b);
-        if (e == hipSuccess) e = hipEventRecord(ev.a, stream);
-        if (e == hipSuccess) e = launch();
-        if (e == hipSuccess) e = hipEventRecord(ev.b, stream);
-        if (e == hipSuccess) evlog->push_back(ev);
-        return e;
-    }
-    int64_t steps = 0;
-    double last_dt = 0.0;
-    double time = 0.0, prev_output_time = 0.0;
-
-    bool fp32c() const { return opt.precision_form == MH_FORM_FP32C; }
-    int fail(int code, const std::string& msg) {
-        err = msg;
-        return code;
-    }
-    int hip_fail(hipError_t e, const char* what) {
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return MH_ERR_HIP;
-    }
-    FormF64::Planes p64(int k) const { return FormF64::Planes{q64[k]}; }
-    FormF32c::Planes p32(int k) const { return FormF32c::Planes{qhi[k], qlo[k]}; }
-
-    GhostGrid ghost_grid(int reduce) const {
-        GhostGrid g{};
-        g.xc = sz.xc;
-        g.yc = sz.yc;
-        g.ng = sz.ng;
-        g.bc[0] = bdry.xs;
-        g.bc[1] = bdry.xe;
-        g.bc[2] = bdry.ys;
-        g.bc[3] = bdry.ye;
-        std::memcpy(g.bconst[0], bdry.xs_const, sizeof(double) * 5);
-        std::memcpy(g.bconst[1], bdry.xe_const, sizeof(double) * 5);
-        std::memcpy(g.bconst[2], bdry.ys_const, sizeof(double) * 5);
-        std::memcpy(g.bconst[3], bdry.ye_const, sizeof(double) * 5);
-        g.gamma = gamma;
-        g.reduce_dt = reduce;
-        return g;
-    }
-
-    hipError_t ghost(int in, int out, int reduce) {
-        unsigned long long* amax = &clock->amax_bits;
-        return timed(1, [&] {
-            if (fp32c()) return launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream);
-            return launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
-        });
-    }
-
-    hipError_t finalize(int advance) {
-        return timed(2, [&] { return launch_clock_finalize(clock, advance, stream); });
-    }
-
-    hipError_t cfl_all(int k) {
-        unsigned long long* amax = &clock->amax_bits;
-        if (fp32c()) return launch_cfl_f32c(nd, p32(k), (int)ncells, gamma, amax, stream);
-        return launch_cfl_f64(nd, p64(k), (int)ncells, gamma, amax, stream);
-    }
-
-    // One stage from buffer `in` into `out` with Q^n in `qn_k`, then ghost fill.
-    hipError_t stage(int in, int qn_k, int out, int kind, int reduce) {
-        hipError_t e = timed(0, [&] { return stage_kernel(in, qn_k, out, kind, reduce); });
-        if (e != hipSuccess) return e;
-        return ghost(in, out, reduce);
-    }
-
-    hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce) {
-        hipError_t e;
-        if (fp32c()) {
-            FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, gamma, gamma - 1.0,
-                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor};
-            if (nd == 2) {
-                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
-                e = launch_stage2d_f32c(sch, g, p, stream);
-            } else {
-                e = launch_stage1d_f32c(sch, kind, reduce, sz.xc, sz.ng, p, stream);
-            }
-        } else {
-            FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, &clock->amax_bits, nullptr};
-            if (nd == 2) {
-                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
-                e = launch_stage2d_f64(sch, g, p, stream);
-            } else {
-                e = launch_stage1d_f64(sch, kind, reduce, sz.xc, sz.ng, p, stream);
-            }
-        }
-        return e;
-    }
-
-    // All stages of one step starting from `cur`; returns the new cur.
-    int enqueue_step(int k, bool reduce_last, hipError_t& e) {
-        const int ns = num_stages(sch.stepper);
-        int in = k;
-        for (int s = 0; s < ns; ++s) {
-            const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
-            e = stage(in, k, out, stage_kind(sch.stepper, s), (reduce_last && s == ns - 1) ? 1 : 0);
-            if (e != hipSuccess) return k;
-            in = out;
-        }
-        return in;
-    }
-
-    void free_all() {
-        for (int k = 0; k < 3; ++k) {
-            if (graph[k]) hipGraphExecDestroy(graph[k]);
-            graph[k] = nullptr;
-            if (q64[k]) hipFree(q64[k]);
-            if (qhi[k]) hipFree(qhi[k]);
-            if (qlo[k]) hipFree(qlo[k]);
-            q64[k] = nullptr;
-            qhi[k] = qlo[k] = nullptr;
-        }
-        for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
-        ev_pool.clear();
-        if (staging) hipFree(staging);
-        if (clock) hipFree(clock);
-        if (floor_hits) hipFree(floor_hits);
-        staging = nullptr;
-        clock = nullptr;
-        floor_hits = nullptr;
-        if (stream) hipStreamDestroy(stream);
-        stream = nullptr;
-    }
-};
 
 extern "C" {
 

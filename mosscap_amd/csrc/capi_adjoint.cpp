@@ -1,0 +1,172 @@
+// C ABI of the discrete adjoint (mh_adjoint_*): checkpointed forward with
+// recorded dt, then the reverse sweep segment by segment (recompute the
+// segment's stage states from its checkpoint, then run the stages backwards).
+#include <vector>
+
+#include "ctx.hpp"
+#include "launch_adjoint.hpp"
+
+namespace {
+
+// Stage k maps (Q_s, Q^n) to a_qn*Q^n + a_qs*Q_s + a_L*L(Q_s) (TimeStepping.cpp stage lambdas).
+struct StageCoef {
+    double a_qn, a_qs, a_l;
+};
+
+StageCoef coef_of(int kind) {
+    switch (kind) {
+        case kAddFull: return {0.0, 1.0, 1.0};
+        case kRk2Final: return {0.5, 0.5, 0.5};
+        case kSsp3Second: return {0.75, 0.25, 0.25};
+        case kSsp3Third: return {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0};
+        case kAddHalf: return {0.0, 1.0, 0.5};
+        default: return {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    }
+}
+
+struct DevVec {
+    double* p = nullptr;
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(double)); }
+    ~DevVec() {
+        if (p) hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double max_cfl, const double* dts_in,
+                       double* dts_out) {
+    if (!ctx || nsteps < 1 || checkpoint_every < 1) return MH_ERR_INVALID;
+    if (ctx->fp32c() || ctx->nd != 2) return ctx->fail(MH_ERR_UNSUPPORTED, "the adjoint runs on FP64 2-D contexts");
+    const int32_t bcs[4] = {ctx->bdry.xs, ctx->bdry.xe, ctx->bdry.ys, ctx->bdry.ye};
+    for (int32_t b : bcs)
+        if (b == (int32_t)BoundaryType::SymmetricOutflowDiode)
+            return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint for symmetricoutflowdiode boundaries");
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
+    hipError_t e = hipSetDevice(ctx->device);
+    ctx->free_adjoint();
+    const int nck = (int)((nsteps + checkpoint_every - 1) / checkpoint_every);
+    const size_t bytes = (size_t)ctx->nvals * sizeof(double);
+    ctx->ckpt.assign(nck, nullptr);
+    for (int k = 0; k < nck && e == hipSuccess; ++k) e = hipMalloc(&ctx->ckpt[k], bytes);
+    if (e != hipSuccess) {
+        ctx->free_adjoint();
+        return ctx->hip_fail(e, "mh_adjoint_forward: checkpoints");
+    }
+    ctx->adj_every = checkpoint_every;
+    ctx->adj_dts.resize(nsteps);
+    for (int64_t n = 0; n < nsteps; ++n) {
+        if (n % checkpoint_every == 0) {
+            e = hipMemcpyAsync(ctx->ckpt[n / checkpoint_every], ctx->q64[ctx->cur], bytes, hipMemcpyDeviceToDevice,
+                               ctx->stream);
+            if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_forward");
+        }
+        double dt = 0.0;
+        if (dts_in) {
+            dt = dts_in[n];
+        } else {
+            const int rc = mh_compute_dt(ctx, max_cfl, 0.0, 1e300, 0.0, 1e300, &dt);
+            if (rc != MH_OK) return rc;
+        }
+        ctx->adj_dts[n] = dt;
+        const int rc = mh_time_step(ctx, dt);
+        if (rc != MH_OK) return rc;
+    }
+    if (dts_out)
+        for (int64_t n = 0; n < nsteps; ++n) dts_out[n] = ctx->adj_dts[n];
+    return MH_OK;
+}
+
+int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_initial) {
+    if (!ctx || !lambda_final || !lambda_initial) return MH_ERR_INVALID;
+    const int64_t nsteps = (int64_t)ctx->adj_dts.size();
+    if (nsteps == 0 || ctx->ckpt.empty()) return ctx->fail(MH_ERR_INVALID, "run mh_adjoint_forward first");
+    hipError_t e = hipSetDevice(ctx->device);
+    const int ns = num_stages(ctx->sch.stepper);
+    const int every = ctx->adj_every;
+    const size_t nv = (size_t)ctx->nvals;
+    AdjGrid g{};
+    g.xc = ctx->sz.xc;
+    g.yc = ctx->sz.yc;
+    g.ng = ctx->sz.ng;
+    g.nx = g.xc - 2 * g.ng;
+    g.ny = g.yc - 2 * g.ng;
+    g.ncells = (int)ctx->ncells;
+    g.bc[0] = ctx->bdry.xs;
+    g.bc[1] = ctx->bdry.xe;
+    g.bc[2] = ctx->bdry.ys;
+    g.bc[3] = ctx->bdry.ye;
+    g.gamma = ctx->gamma;
+    // scratch: primitives, two face-cotangent arrays, two 20-plane contribution arrays
+    DevVec w, lrr, lrl, gx, gy, mu, acc, cur, nxt, t;
+    for (DevVec* v : {&w, &lrr, &lrl, &mu, &acc, &cur, &nxt, &t})
+        if (e == hipSuccess) e = v->alloc(nv);
+    if (e == hipSuccess) e = gx.alloc(5 * nv);
+    if (e == hipSuccess) e = gy.alloc(5 * nv);
+    // stage states of one segment: (every) steps x ns stages
+    std::vector<DevVec> seg((size_t)every * ns);
+    for (auto& v : seg)
+        if (e == hipSuccess) e = v.alloc(nv);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_backward: scratch");
+    const AdjScratch sc{w.p, lrr.p, lrl.p, gx.p, gy.p};
+    if (e == hipSuccess) e = hipMemcpyAsync(mu.p, lambda_final, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, mu.p, 0.0, mu.p, ctx->stream);  // clear ghost entries
+    const int nseg = (int)ctx->ckpt.size();
+    for (int sgi = nseg - 1; sgi >= 0 && e == hipSuccess; --sgi) {
+        const int64_t n0 = (int64_t)sgi * every;
+        const int64_t n1 = std::min<int64_t>(n0 + every, nsteps);
+        // recompute the segment: seg[(n-n0)*ns + s] holds the input of stage s of step n
+        e = hipMemcpyAsync(seg[0].p, ctx->ckpt[sgi], nv * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+        for (int64_t n = n0; n < n1 && e == hipSuccess; ++n) {
+            e = launch_clock_set_dt(ctx->clock, ctx->adj_dts[n], ctx->stream);
+            const double* qn = seg[(n - n0) * ns].p;
+            for (int s = 0; s < ns && e == hipSuccess; ++s) {
+                const double* in = seg[(n - n0) * ns + s].p;
+                const bool last = (s == ns - 1);
+                if (last && n + 1 == n1) break;  // the segment's end state is not needed
+                double* out = last ? seg[(n + 1 - n0) * ns].p : seg[(n - n0) * ns + s + 1].p;
+                e = ctx->stage_ptr(in, qn, out, stage_kind(ctx->sch.stepper, s));
+            }
+        }
+        // reverse the segment's steps
+        for (int64_t n = n1 - 1; n >= n0 && e == hipSuccess; --n) {
+            g.dtdx = ctx->adj_dts[n] / ctx->dx;
+            // cur = adjoint of the last stage's output; acc = adjoint of Q^n through the Q^n terms
+            e = launch_adjoint_axpby(g, 1.0, mu.p, 0.0, cur.p, ctx->stream);
+            if (e == hipSuccess) e = launch_adjoint_axpby(g, 0.0, mu.p, 0.0, acc.p, ctx->stream);
+            for (int s = ns - 1; s >= 0 && e == hipSuccess; --s) {
+                const StageCoef c = coef_of(stage_kind(ctx->sch.stepper, s));
+                e = launch_adjoint_jt(ctx->sch, g, seg[(n - n0) * ns + s].p, cur.p, t.p, sc, ctx->stream);
+                if (e == hipSuccess && c.a_qn != 0.0) e = launch_adjoint_axpby(g, c.a_qn, cur.p, 1.0, acc.p, ctx->stream);
+                if (e == hipSuccess) e = launch_adjoint_axpby(g, c.a_qs, cur.p, 0.0, nxt.p, ctx->stream);
+                if (e == hipSuccess) e = launch_adjoint_axpby(g, c.a_l, t.p, 1.0, nxt.p, ctx->stream);
+                std::swap(cur.p, nxt.p);
+            }
+            // dJ/dQ^n = acc + cur (stage 0's input is Q^n)
+            if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, acc.p, 1.0, cur.p, ctx->stream);
+            std::swap(mu.p, cur.p);
+        }
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lambda_initial, mu.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_adjoint_backward");
+}
+
+int mh_probe_vjp(int32_t what, int64_t n, const double* in, const double* cot, double* out, double gamma) {
+    if (n < 1 || n > (1 << 24) || !in || !cot || !out || what < 0 || what > 5) return MH_ERR_INVALID;
+    const int nin = what < 2 ? 5 : 8, ncot = what < 2 ? 2 : 4;
+    DevVec a, b, c;
+    hipError_t e = a.alloc(n * nin);
+    if (e == hipSuccess) e = b.alloc(n * ncot);
+    if (e == hipSuccess) e = c.alloc(n * nin);
+    if (e == hipSuccess) e = hipMemcpy(a.p, in, n * nin * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.p, cot, n * ncot * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_vjp(what, (int)n, a.p, b.p, c.p, gamma, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, c.p, n * nin * sizeof(double), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MH_OK : MH_ERR_HIP;
+}
+
+}  // extern "C"
