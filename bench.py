@@ -88,6 +88,35 @@ def parity(M, cfg, q0, nsteps):
     return _rel_l1(out[M.FP32C], out[M.FP64], cfg.ng)
 
 
+def adjoint_ratio(M, problems, solver, n=512, nsteps=200, every=20):
+    """KH n^2 (WENO5-Z + `solver` + SSPRK3, FP64), nsteps frozen-dt steps, checkpoint every `every`:
+    device time of the reverse sweep (segment recomputation + reverse stages) over the wall time
+    of the same nsteps forward steps replayed from a graph."""
+    from mosscap_amd.adjoint import KHAdjointProblem
+    prob = KHAdjointProblem(n=n, nsteps=nsteps, checkpoint_every=every, riemann_solver=solver)
+    control = 0.01 * (2 * problems.uniform01(17, np.arange(n * n)).reshape(n, n) - 1)
+    target = prob.forward(0.01 * (2 * problems.uniform01(18, np.arange(n * n)).reshape(n, n) - 1))
+    rho = prob.forward(control)
+    diff = rho - target
+    prob.backward(diff / np.linalg.norm(diff))
+    bwd_ms = prob.ctx.adjoint_backward_ms()
+    q0 = prob.initial_state(control)
+    with M.HydroContext(prob.cfg, precision_form=M.FP64) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        ctx.run(3, use_graph=True)
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        _sync()
+        t0 = time.perf_counter()
+        ctx.run(nsteps, use_graph=True)
+        _sync()
+        fwd_ms = (time.perf_counter() - t0) * 1e3
+    prob.ctx.close()
+    return {"solver": solver, "grid": n, "steps": nsteps, "checkpoint_every": every,
+            "backward_ms": round(bwd_ms, 3), "forward_ms": round(fwd_ms, 3), "ratio": round(bwd_ms / fwd_ms, 3)}
+
+
 def cpu_baseline(cfg, q0, seconds):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -274,6 +303,7 @@ def main(argv=None):
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one (profiler passes)")
     ap.add_argument("--span-rows", type=int, default=0, help="rows per wavefront (0: autotuned)")
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in span heuristic")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
@@ -312,6 +342,7 @@ def main(argv=None):
     v64 = cells * a.steps / el64 / 1e6
     v32 = cells * a.steps / el32 / 1e6
     rel = parity(M, cfg, q0, a.parity_steps) if a.parity_steps > 0 else None
+    adj = [adjoint_ratio(M, problems, sv) for sv in ("rusanov", "hllc")] if not a.no_adjoint else None
 
     # dominant kernel: the fused stage; 256 algorithmic bytes per cell-step over 3 launches
     bytes_per_launch = 256.0 * cells / 3.0
@@ -345,7 +376,8 @@ def main(argv=None):
         "fp32c_over_fp64": round(v32 / v64, 3),
         "rel_l1_fp32c_vs_fp64": None if rel is None else {"steps": a.parity_steps,
                                                             "rho_mx_my_E": [float(f"{r:.3e}") for r in rel]},
-        "adjoint_over_forward": None,
+        "adjoint_over_forward": None if adj is None else adj[0]["ratio"],
+        "adjoint": adj,
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs32 / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("void mh::stage2d_kernel<mh::FormF32c, (mh::Reconstruction)3, "

@@ -170,6 +170,10 @@ void* mh_stream(mh_ctx* ctx);
 int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double max_cfl, const double* dts_in,
                        double* dts_out);
 int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_initial);
+/* Device time (HIP events on the context's stream) of the last
+ * mh_adjoint_backward: segment recomputation plus reverse stages, the two
+ * host transfers excluded. */
+int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms);
 /* probe of the hand-written VJPs against dual-number Jacobians (what: 0/1 WENO5-Z
  * hand/dual, in[n][5], cot[n][2] -> out[n][5]; 2/3 Rusanov x hand/dual, 4/5 Rusanov y,
  * in[n][8] (wL, wR), cot[n][4] -> out[n][8]). */

@@ -104,6 +104,7 @@ _SIGNATURES = [
     ("mh_stream", C.c_void_p, [C.c_void_p]),
     ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_adjoint_backward_ms", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mh_probe_vjp", C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     ("mh_batch_create", C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.POINTER(MhBoundaries),
                                   C.POINTER(MhSchemes), C.c_double, C.POINTER(MhOptions), C.POINTER(C.c_void_p)]),
@@ -323,6 +324,12 @@ class HydroContext:
         out = np.empty(self.cfg.shape)
         self._c(lib().mh_adjoint_backward(self._h, _ptr(lf), _ptr(out)))
         return out
+
+    def adjoint_backward_ms(self) -> float:
+        """Device milliseconds of the last adjoint_backward (recompute + reverse, transfers excluded)."""
+        ms = C.c_double()
+        self._c(lib().mh_adjoint_backward_ms(self._h, C.byref(ms)))
+        return ms.value
 
     @property
     def stream(self) -> int:

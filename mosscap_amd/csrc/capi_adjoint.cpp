@@ -113,6 +113,10 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     const AdjScratch sc{w.p, lrr.p, lrl.p, gx.p, gy.p};
     if (e == hipSuccess) e = hipMemcpyAsync(mu.p, lambda_final, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, mu.p, 0.0, mu.p, ctx->stream);  // clear ghost entries
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    if (e == hipSuccess) e = hipEventRecord(ev0, ctx->stream);
     const int nseg = (int)ctx->ckpt.size();
     for (int sgi = nseg - 1; sgi >= 0 && e == hipSuccess; --sgi) {
         const int64_t n0 = (int64_t)sgi * every;
@@ -149,10 +153,22 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
             std::swap(mu.p, cur.p);
         }
     }
+    if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(lambda_initial, mu.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+    ctx->adj_bwd_ms = e == hipSuccess ? (double)ms : -1.0;
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
     return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_adjoint_backward");
+}
+
+int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms) {
+    if (!ctx || !ms) return MH_ERR_INVALID;
+    *ms = ctx->adj_bwd_ms;
+    return MH_OK;
 }
 
 int mh_probe_vjp(int32_t what, int64_t n, const double* in, const double* cot, double* out, double gamma) {

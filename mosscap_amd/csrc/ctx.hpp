@@ -181,6 +181,7 @@ struct mh_ctx {
     std::vector<double*> ckpt;
     std::vector<double> adj_dts;
     int adj_every = 0;
+    double adj_bwd_ms = 0.0;  // device time of the last reverse sweep
     void free_adjoint() {
         for (double* p : ckpt)
             if (p) hipFree(p);
