@@ -97,7 +97,7 @@ int mh_download(mh_ctx* ctx, double* q_host);
 int mh_upload_device(mh_ctx* ctx, const double* q_dev);
 int mh_download_device(mh_ctx* ctx, double* q_dev);
 
-/* fill_bcs(sim), Boundaries.hpp:132-149 (user_bc is the caller's). */
+/* fill_bcs(sim), Boundaries.hpp:132-149; runs the user_bc hook (below) when set. */
 int mh_fill_bcs(mh_ctx* ctx);
 
 /* compute_dt(sim), Hydro.cpp:208-242: CFL reduction over all cells incl.
@@ -150,6 +150,32 @@ int mh_get_stats(mh_ctx* ctx, mh_stats* out);
 
 /* HIP stream the context launches on (a hipStream_t), for event timing. */
 void* mh_stream(mh_ctx* ctx);
+
+/* ---- user boundary hook: Simulation::user_bc (Simulation.hpp:55), called at
+ * the end of fill_bcs (Boundaries.hpp:146-148) ----------------------------------
+ * The hook runs on the calling host thread after every ghost fill of the
+ * state: mh_fill_bcs, each stage of mh_time_step / mh_stage / mh_run. Inside
+ * it, mh_device_state gives the buffer just filled; the hook may enqueue
+ * kernels on mh_stream(ctx) or copy synchronously. With the compensated form
+ * it must write both planes (hi = (float)v, lo = (float)(v - hi)). A non-zero
+ * return fails the calling function with MH_ERR_INVALID. While a hook is set,
+ * the CFL reduction runs as a separate pass after it (so the hook's ghost
+ * values enter dt, as in Hydro.cpp:214-228), mh_run launches step by step
+ * instead of replaying a graph, and the adjoint is refused. fn = NULL removes
+ * the hook. */
+typedef int (*mh_user_bc_fn)(mh_ctx* ctx, void* user);
+int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user);
+
+typedef struct {
+    int32_t precision_form; /* MH_FORM_* */
+    double* q;              /* FP64 state, or NULL */
+    float* q_hi;            /* compensated state planes, or NULL */
+    float* q_lo;
+    int64_t num_values;     /* values per plane: nvar * zc * yc * xc */
+} mh_device_view;
+/* Device pointers of the state buffer the hook must act on (inside a hook) or
+ * of the current state (elsewhere); laid out like Q. */
+int mh_device_state(mh_ctx* ctx, mh_device_view* out);
 
 /* ---- discrete adjoint (no counterpart in the reference) ----------------------
  * FP64 2-D contexts; every boundary type but symmetricoutflowdiode. Time step

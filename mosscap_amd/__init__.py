@@ -62,6 +62,14 @@ class MhStats(C.Structure):
                 ("prev_output_time", C.c_double)]
 
 
+class MhDeviceView(C.Structure):
+    _fields_ = [("precision_form", C.c_int32), ("q", C.c_void_p), ("q_hi", C.c_void_p), ("q_lo", C.c_void_p),
+                ("num_values", C.c_int64)]
+
+
+USER_BC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+
+
 class MhTiming(C.Structure):
     _fields_ = [("stage_ms", C.c_double), ("stage_launches", C.c_int64), ("ghost_ms", C.c_double),
                 ("ghost_launches", C.c_int64), ("clock_ms", C.c_double), ("clock_launches", C.c_int64),
@@ -102,6 +110,8 @@ _SIGNATURES = [
     ("mh_autotune", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     ("mh_get_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
     ("mh_stream", C.c_void_p, [C.c_void_p]),
+    ("mh_set_user_bc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_device_state", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward_ms", C.c_int, [C.c_void_p, C.c_void_p]),
@@ -334,6 +344,30 @@ class HydroContext:
     @property
     def stream(self) -> int:
         return lib().mh_stream(self._h) or 0
+
+    def set_user_bc(self, fn) -> None:
+        """Install fn(view: MhDeviceView) -> None as the user_bc hook (None removes it). It runs after
+        every ghost fill with the device pointers of the buffer just filled; an exception in fn
+        fails the library call that ran it."""
+        if fn is None:
+            self._c(lib().mh_set_user_bc(self._h, None, None))
+            self._user_bc = None
+            return
+
+        def hook(h, _user):
+            try:
+                v = MhDeviceView()
+                if lib().mh_device_state(h, C.byref(v)) != 0:
+                    return 1
+                fn(v)
+                return 0
+            except Exception:  # reported through the library's return code
+                import traceback
+                traceback.print_exc()
+                return 1
+
+        self._user_bc = USER_BC_FN(hook)  # keep the trampoline alive
+        self._c(lib().mh_set_user_bc(self._h, C.cast(self._user_bc, C.c_void_p), None))
 
 
 class BatchContext:

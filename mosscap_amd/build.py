@@ -5,6 +5,8 @@
 
 Outputs (git-ignored, shipped to the GPU box with the tree):
     mosscap_amd/libmosscap_hip.so   the product: kernels + C ABI (include/mosscap_hip.h)
+    mosscap_amd/libmosscap_host.so  host layer: YAML setup, problems, output, driver (include/mosscap_host.h)
+    mosscap_amd/mosscap_hip         command-line driver (the reference's main.cpp)
     oracle/_build/libmh_oracle.so   the CPU oracle (test infrastructure)
 
 Every HIP translation unit is compiled with packed-FP32 instructions
@@ -28,6 +30,9 @@ PKG = ROOT / "mosscap_amd"
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libmosscap_hip.so"
+HOST_LIB = PKG / "libmosscap_host.so"
+HOST_EXE = PKG / "mosscap_hip"
+HOST_UNITS = ["yaml_lite.cpp", "setup.cpp", "problems.cpp", "output_nc.cpp", "host_capi.cpp"]
 ORACLE_SRC = ROOT / "oracle" / "mh_oracle.cpp"
 ORACLE_LIB = ROOT / "oracle" / "_build" / "libmh_oracle.so"
 
@@ -85,6 +90,33 @@ def build_lib(jobs: int = 4, defines: list[str] | None = None, out: Path | None 
     return lib
 
 
+def _git_hash() -> str:
+    try:
+        r = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short", "HEAD"], stdout=subprocess.PIPE,
+                           stderr=subprocess.DEVNULL, text=True)
+        return r.stdout.strip() or "unknown"
+    except OSError:
+        return "unknown"
+
+
+def build_host() -> Path:
+    """The host layer (plain C++, no device code) linked against the HIP library,
+    and the command-line driver. -ffp-contract=off keeps the initial-condition
+    and output arithmetic identical to the Python generators."""
+    cxx = shutil.which("g++") or "g++"
+    flags = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{ROOT / 'include'}",
+             f'-DMOSSCAP_GIT_HASH="{_git_hash()}"']
+    tmp = HOST_LIB.with_suffix(".so.tmp")
+    _run([cxx, *flags, "-shared", *[str(CSRC / "host" / u) for u in HOST_UNITS], "-o", str(tmp),
+          f"-L{PKG}", "-lmosscap_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    tmp = HOST_EXE.with_suffix(".tmp")
+    _run([cxx, *flags, str(CSRC / "host" / "main.cpp"), "-o", str(tmp), f"-L{PKG}", "-lmosscap_host",
+          "-lmosscap_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_EXE)
+    return HOST_LIB
+
+
 def build_oracle() -> Path:
     ORACLE_LIB.parent.mkdir(parents=True, exist_ok=True)
     cxx = shutil.which("g++") or "g++"
@@ -105,6 +137,8 @@ def main(argv: list[str] | None = None) -> int:
     a = ap.parse_args(argv)
     if not a.oracle:
         print("built", build_lib(a.j, a.defines, a.out), flush=True)
+        if a.out is None:
+            print("built", build_host(), flush=True)
     if not a.lib:
         print("built", build_oracle(), flush=True)
     return 0

@@ -145,6 +145,29 @@ int64_t mh_num_values(const mh_ctx* ctx) { return ctx ? ctx->nvals : 0; }
 
 void* mh_stream(mh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user) {
+    if (!ctx) return MH_ERR_INVALID;
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
+    ctx->user_fn = fn;
+    ctx->user_data = fn ? user : nullptr;
+    return MH_OK;
+}
+
+int mh_device_state(mh_ctx* ctx, mh_device_view* out) {
+    if (!ctx || !out) return MH_ERR_INVALID;
+    const int k = ctx->view_buf >= 0 ? ctx->view_buf : ctx->cur;
+    *out = mh_device_view{};
+    out->precision_form = ctx->opt.precision_form;
+    out->num_values = ctx->nvals;
+    if (ctx->fp32c()) {
+        out->q_hi = ctx->qhi[k];
+        out->q_lo = ctx->qlo[k];
+    } else {
+        out->q = ctx->q64[k];
+    }
+    return MH_OK;
+}
+
 static int upload_impl(mh_ctx* c, const double* q, hipMemcpyKind kind) {
     if (!c || !q) return MH_ERR_INVALID;
     hipError_t e = hipSetDevice(c->device);
@@ -264,7 +287,7 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
     if (e == hipSuccess) e = ctx->cfl_all(ctx->cur);
     for (int64_t s = 0; s < nsteps && e == hipSuccess; ++s) {
         const int k = ctx->cur;
-        if (use_graph) {
+        if (use_graph && !ctx->user_fn) {
             if (!ctx->graph[k]) {
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);

@@ -41,9 +41,13 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
     if (!ctx || nsteps < 1 || checkpoint_every < 1) return MH_ERR_INVALID;
     if (ctx->fp32c() || ctx->nd != 2) return ctx->fail(MH_ERR_UNSUPPORTED, "the adjoint runs on FP64 2-D contexts");
     const int32_t bcs[4] = {ctx->bdry.xs, ctx->bdry.xe, ctx->bdry.ys, ctx->bdry.ye};
-    for (int32_t b : bcs)
+    for (int32_t b : bcs) {
         if (b == (int32_t)BoundaryType::SymmetricOutflowDiode)
             return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint for symmetricoutflowdiode boundaries");
+        if (b == (int32_t)BoundaryType::UserFn)
+            return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint for userfn boundaries");
+    }
+    if (ctx->user_fn) return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint while a user_bc hook is set");
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
     ctx->free_adjoint();

@@ -84,6 +84,11 @@ struct mh_ctx {
         if (e == hipSuccess) evlog->push_back(ev);
         return e;
     }
+    // user_bc hook (mh_set_user_bc)
+    mh_user_bc_fn user_fn = nullptr;
+    void* user_data = nullptr;
+    int view_buf = -1;  // buffer the running hook acts on
+    int hook_rc = 0;    // non-zero return of the last hook call
     int64_t steps = 0;
     double last_dt = 0.0;
     double time = 0.0, prev_output_time = 0.0;
@@ -94,6 +99,11 @@ struct mh_ctx {
         return code;
     }
     int hip_fail(hipError_t e, const char* what) {
+        if (hook_rc != 0) {
+            err = std::string(what) + ": the user_bc hook returned " + std::to_string(hook_rc);
+            hook_rc = 0;
+            return MH_ERR_INVALID;
+        }
         err = std::string(what) + ": " + hipGetErrorString(e);
         return MH_ERR_HIP;
     }
@@ -120,10 +130,15 @@ struct mh_ctx {
 
     hipError_t ghost(int in, int out, int reduce) {
         unsigned long long* amax = &clock->amax_bits;
-        return timed(1, [&] {
+        hipError_t e = timed(1, [&] {
             if (fp32c()) return launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream);
             return launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
         });
+        if (e != hipSuccess || !user_fn) return e;
+        view_buf = out;
+        hook_rc = user_fn(this, user_data);
+        view_buf = -1;
+        return hook_rc != 0 ? hipErrorUnknown : hipGetLastError();
     }
 
     hipError_t finalize(int advance) {
@@ -190,15 +205,18 @@ struct mh_ctx {
     }
 
     // All stages of one step starting from `cur`; returns the new cur.
+    // With a user_bc hook the reduction is a separate pass after the hook.
     int enqueue_step(int k, bool reduce_last, hipError_t& e) {
         const int ns = num_stages(sch.stepper);
+        const bool fused = reduce_last && !user_fn;
         int in = k;
         for (int s = 0; s < ns; ++s) {
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
-            e = stage(in, k, out, stage_kind(sch.stepper, s), (reduce_last && s == ns - 1) ? 1 : 0);
+            e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0);
             if (e != hipSuccess) return k;
             in = out;
         }
+        if (reduce_last && user_fn) e = cfl_all(in);
         return in;
     }
 
