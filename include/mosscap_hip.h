@@ -25,7 +25,7 @@ extern "C" {
 
 #define MH_OK 0
 #define MH_ERR_INVALID 1     /* bad argument; the reference throws std::runtime_error (SetupModules.hpp:146) */
-#define MH_ERR_UNSUPPORTED 2 /* valid for the reference but outside this library (e.g. 3-D, tracers) */
+#define MH_ERR_UNSUPPORTED 2 /* valid for the reference but outside this library (e.g. 3-D, 1-D tracers) */
 #define MH_ERR_HIP 3         /* HIP runtime or launch failure */
 
 #define MH_FORM_FP64 0  /* drop-in: FP64 state and arithmetic, bitwise the reference CPU build */
@@ -79,7 +79,10 @@ typedef struct {
 
 /* Build a context (replaces setup_grid's allocations, SetupModules.hpp:9-64,
  * plus select_hydro_fns / select_timestepper, Hydro.cpp:244-323 and
- * TimeStepping.cpp:331-344). num_dim 1 or 2; num_tracers must be 0. */
+ * TimeStepping.cpp:331-344). num_dim 1 or 2. num_tracers (simulation.n_extra_fields,
+ * 0..16, 2-D only) passive fields follow the hydro variables in Q: conserved
+ * tracer densities, advected with the upwinded density flux (Hydro.cpp:27-33,
+ * 155-162); constant boundaries are refused with tracers. */
 int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const mh_boundaries* bdry,
               int32_t num_dim, int32_t num_tracers, const mh_schemes* schemes, double gamma,
               const mh_options* opts, mh_ctx** out);
@@ -202,7 +205,7 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
 int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms);
 /* probe of the hand-written VJPs against dual-number Jacobians (what: 0/1 WENO5-Z
  * hand/dual, in[n][5], cot[n][2] -> out[n][5]; 2/3 Rusanov x hand/dual, 4/5 Rusanov y,
- * in[n][8] (wL, wR), cot[n][4] -> out[n][8]). */
+ * 6/7 HLLC x, 8/9 HLLC y: in[n][8] (wL, wR), cot[n][4] -> out[n][8]). */
 int mh_probe_vjp(int32_t what, int64_t n, const double* in, const double* cot, double* out, double gamma);
 
 /* ---- batched ensemble of independent 1-D problems -------------------------

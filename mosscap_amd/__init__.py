@@ -186,6 +186,7 @@ class Config:
     max_cfl: float = 0.8
     max_time: float = 1.0
     output_delta: float = 0.1
+    num_tracers: int = 0  # simulation.n_extra_fields
 
     @property
     def xc(self) -> int:
@@ -197,7 +198,7 @@ class Config:
 
     @property
     def nvar(self) -> int:
-        return 2 + self.num_dim
+        return 2 + self.num_dim + self.num_tracers
 
     @property
     def dx(self) -> float:
@@ -228,7 +229,8 @@ class HydroContext:
                         RIEMANN_SOLVER[cfg.riemann_solver], TIME_STEPPER[cfg.time_stepper])
         opt = MhOptions(precision_form, span_rows, rho_floor, p_floor)
         h = C.c_void_p()
-        _check(L.mh_create(C.byref(sz), cfg.dx, C.byref(loc), C.byref(b), cfg.num_dim, 0, C.byref(sch), cfg.gamma,
+        _check(L.mh_create(C.byref(sz), cfg.dx, C.byref(loc), C.byref(b), cfg.num_dim, cfg.num_tracers, C.byref(sch),
+                           cfg.gamma,
                            C.byref(opt), C.byref(h)))
         self._h = h
         self.n_values = L.mh_num_values(h)

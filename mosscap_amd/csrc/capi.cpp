@@ -34,7 +34,9 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
         return bad(MH_ERR_INVALID, "simulation.num_dim = " + std::to_string(num_dim) + ", must be in range [1, 3]");
     }
     if (num_dim == 3) return bad(MH_ERR_UNSUPPORTED, "3-D grids are not built in this library");
-    if (num_tracers != 0) return bad(MH_ERR_UNSUPPORTED, "tracer fields (simulation.n_extra_fields) are not built");
+    if (num_tracers < 0 || num_tracers > 16) return bad(MH_ERR_INVALID, "num_tracers must be in [0, 16]");
+    if (num_tracers > 0 && num_dim != 2)
+        return bad(MH_ERR_UNSUPPORTED, "tracer fields (simulation.n_extra_fields) are built for 2-D grids only");
     if (schemes->reconstruction < 0 || schemes->reconstruction > 3 || schemes->slope_limit < 0 ||
         schemes->slope_limit > 2 || schemes->riemann_solver < 0 || schemes->riemann_solver > 2 ||
         schemes->time_stepper < 0 || schemes->time_stepper > 2) {
@@ -64,13 +66,20 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
         return bad(MH_ERR_INVALID, "periodic boundaries must be paired (ys and ye)");
     }
     if (!(dx > 0.0) || !(gamma > 1.0)) return bad(MH_ERR_INVALID, "dx must be > 0 and gamma > 1");
+    if (num_tracers > 0) {
+        for (int k = 0; k < 4; ++k)
+            if (bcs[k] == (int32_t)BoundaryType::Constant)
+                // the reference's constant vector holds the 5 hydro values only (State.hpp:93-98)
+                return bad(MH_ERR_UNSUPPORTED, "constant boundaries with tracer fields");
+    }
     const int64_t ncells = (int64_t)sz->xc * sz->yc * sz->zc;
-    const int nvar = 2 + num_dim;
+    const int nvar = 2 + num_dim + num_tracers;
     if (ncells * nvar >= (int64_t)1 << 31) return bad(MH_ERR_INVALID, "state larger than 2^31 values");
 
     mh_ctx* c = new mh_ctx();
     c->nd = num_dim;
     c->nvar = nvar;
+    c->ntr = num_tracers;
     c->sz = *sz;
     c->dx = dx;
     if (loc) c->loc = *loc;
@@ -109,6 +118,18 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
         }
     }
     if (e == hipSuccess && c->fp32c()) e = hipMalloc(&c->staging, bytes64);
+    if (e == hipSuccess && num_tracers > 0) {
+        const size_t n = (size_t)ncells;
+        if (c->fp32c()) {
+            e = hipMalloc(&c->frx32, n * sizeof(float));
+            if (e == hipSuccess) e = hipMalloc(&c->fry32, n * sizeof(float));
+            if (e == hipSuccess) e = hipMalloc(&c->wtr32, n * num_tracers * sizeof(float));
+        } else {
+            e = hipMalloc(&c->frx64, n * sizeof(double));
+            if (e == hipSuccess) e = hipMalloc(&c->fry64, n * sizeof(double));
+            if (e == hipSuccess) e = hipMalloc(&c->wtr64, n * num_tracers * sizeof(double));
+        }
+    }
     if (e == hipSuccess) e = hipMalloc(&c->clock, sizeof(Clock));
     if (e == hipSuccess) e = hipMalloc(&c->floor_hits, sizeof(unsigned int));
     if (e == hipSuccess) e = hipMemsetAsync(c->clock, 0, sizeof(Clock), c->stream);

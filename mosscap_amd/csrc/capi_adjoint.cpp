@@ -48,6 +48,7 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
             return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint for userfn boundaries");
     }
     if (ctx->user_fn) return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint while a user_bc hook is set");
+    if (ctx->ntr > 0) return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint of tracer fields");
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
     ctx->free_adjoint();
@@ -103,18 +104,15 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     g.bc[2] = ctx->bdry.ys;
     g.bc[3] = ctx->bdry.ye;
     g.gamma = ctx->gamma;
-    // scratch: primitives, two face-cotangent arrays, two 20-plane contribution arrays
-    DevVec w, lrr, lrl, gx, gy, mu, acc, cur, nxt, t;
-    for (DevVec* v : {&w, &lrr, &lrl, &mu, &acc, &cur, &nxt, &t})
+    // scratch: the x sweep's primitive cotangents and four cotangent states
+    DevVec lamx, mu, acc, cur, nxt;
+    for (DevVec* v : {&lamx, &mu, &acc, &cur, &nxt})
         if (e == hipSuccess) e = v->alloc(nv);
-    if (e == hipSuccess) e = gx.alloc(5 * nv);
-    if (e == hipSuccess) e = gy.alloc(5 * nv);
     // stage states of one segment: (every) steps x ns stages
     std::vector<DevVec> seg((size_t)every * ns);
     for (auto& v : seg)
         if (e == hipSuccess) e = v.alloc(nv);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_backward: scratch");
-    const AdjScratch sc{w.p, lrr.p, lrl.p, gx.p, gy.p};
     if (e == hipSuccess) e = hipMemcpyAsync(mu.p, lambda_final, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, mu.p, 0.0, mu.p, ctx->stream);  // clear ghost entries
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -146,10 +144,8 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
             if (e == hipSuccess) e = launch_adjoint_axpby(g, 0.0, mu.p, 0.0, acc.p, ctx->stream);
             for (int s = ns - 1; s >= 0 && e == hipSuccess; --s) {
                 const StageCoef c = coef_of(stage_kind(ctx->sch.stepper, s));
-                e = launch_adjoint_jt(ctx->sch, g, seg[(n - n0) * ns + s].p, cur.p, t.p, sc, ctx->stream);
-                if (e == hipSuccess && c.a_qn != 0.0) e = launch_adjoint_axpby(g, c.a_qn, cur.p, 1.0, acc.p, ctx->stream);
-                if (e == hipSuccess) e = launch_adjoint_axpby(g, c.a_qs, cur.p, 0.0, nxt.p, ctx->stream);
-                if (e == hipSuccess) e = launch_adjoint_axpby(g, c.a_l, t.p, 1.0, nxt.p, ctx->stream);
+                const AdjStage st{seg[(n - n0) * ns + s].p, cur.p, lamx.p, nxt.p, acc.p, c.a_qn, c.a_qs, c.a_l};
+                e = launch_adjoint_stage(ctx->sch, g, st, ctx->stream);
                 std::swap(cur.p, nxt.p);
             }
             // dJ/dQ^n = acc + cur (stage 0's input is Q^n)
@@ -176,7 +172,7 @@ int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms) {
 }
 
 int mh_probe_vjp(int32_t what, int64_t n, const double* in, const double* cot, double* out, double gamma) {
-    if (n < 1 || n > (1 << 24) || !in || !cot || !out || what < 0 || what > 5) return MH_ERR_INVALID;
+    if (n < 1 || n > (1 << 24) || !in || !cot || !out || what < 0 || what > 9) return MH_ERR_INVALID;
     const int nin = what < 2 ? 5 : 8, ncot = what < 2 ? 2 : 4;
     DevVec a, b, c;
     hipError_t e = a.alloc(n * nin);

@@ -27,6 +27,13 @@ using namespace mh;
 
 struct mh_ctx {
     int nd = 0, nvar = 0;
+    int ntr = 0;                     // tracer planes after the hydro variables
+    double* frx64 = nullptr;         // tracers: density fluxes of the stage (FP64 form)
+    double* fry64 = nullptr;
+    double* wtr64 = nullptr;         // tracers: mass fractions of the stage input
+    float* frx32 = nullptr;          // ... compensated form
+    float* fry32 = nullptr;
+    float* wtr32 = nullptr;
     mh_grid_size sz{};
     double dx = 0.0;
     mh_grid_loc loc{};
@@ -131,8 +138,11 @@ struct mh_ctx {
     hipError_t ghost(int in, int out, int reduce) {
         unsigned long long* amax = &clock->amax_bits;
         hipError_t e = timed(1, [&] {
-            if (fp32c()) return launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream);
-            return launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
+            hipError_t r = fp32c() ? launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream)
+                                   : launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
+            if (r != hipSuccess || ntr == 0) return r;
+            return fp32c() ? launch_tracer_ghost_f32c(ghost_grid(0), ntr, p32(in), p32(out), stream)
+                           : launch_tracer_ghost_f64(ghost_grid(0), ntr, p64(in), p64(out), stream);
         });
         if (e != hipSuccess || !user_fn) return e;
         view_buf = out;
@@ -160,20 +170,24 @@ struct mh_ctx {
 
     hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce) {
         hipError_t e;
+        const TracerGrid tg{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, ntr, kind};
         if (fp32c()) {
             FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, gamma, gamma - 1.0,
-                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor};
+                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor, frx32, fry32};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
                 e = launch_stage2d_f32c(sch, g, p, stream);
+                if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
             } else {
                 e = launch_stage1d_f32c(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
         } else {
-            FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, &clock->amax_bits, nullptr};
+            FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, &clock->amax_bits, nullptr,
+                            frx64, fry64};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
                 e = launch_stage2d_f64(sch, g, p, stream);
+                if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
             } else {
                 e = launch_stage1d_f64(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
@@ -233,6 +247,10 @@ struct mh_ctx {
         }
         for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
         ev_pool.clear();
+        for (void* t : {(void*)frx64, (void*)fry64, (void*)wtr64, (void*)frx32, (void*)fry32, (void*)wtr32})
+            if (t) hipFree(t);
+        frx64 = fry64 = wtr64 = nullptr;
+        frx32 = fry32 = wtr32 = nullptr;
         if (staging) hipFree(staging);
         if (clock) hipFree(clock);
         if (floor_hits) hipFree(floor_hits);

@@ -243,6 +243,184 @@ __host__ __device__ inline void rusanov_vjp2(double gamma, const double (&wL)[4]
     dcs(wR, csR, lwR);
 }
 
+// HLLC (Riemann.hpp:75-176), 2-D along Axis: the forward sequence of
+// riemann_flux<Hllc> replayed, then its reverse sweep; J^T lf accumulated into
+// lwL, lwR. Branches (wave-speed switches, the floors on bM/bP and cp) pass the
+// derivative of the branch the forward values took.
+template <int Axis>
+__host__ __device__ inline void hllc_vjp2(double gamma, const double (&wL)[4], const double (&wR)[4],
+                                          const double (&lf)[4], double (&lwL)[4], double (&lwR)[4]) {
+    constexpr int IV = 1 + Axis;  // normal velocity (prim) = normal momentum (cons) index
+    constexpr double tiny = 1e-20;
+    const double gm1 = gamma - 1.0;
+    const double rL = wL[0], rR = wR[0], pL = wL[3], pR = wR[3];
+    const double uL = wL[IV], uR = wR[IV];
+    // ---- forward (same operations as the functor) ----
+    const double csL = __builtin_sqrt(gamma * pL / rL);
+    const double csR = __builtin_sqrt(gamma * pR / rR);
+    const double v2L = wL[1] * wL[1] + wL[2] * wL[2];
+    const double v2R = wR[1] * wR[1] + wR[2] * wR[2];
+    const double qLm = rL * uL, qRm = rR * uR;
+    const double qLe = pL / gm1 + 0.5 * rL * v2L;
+    const double qRe = pR / gm1 + 0.5 * rR * v2R;
+    const double rho_avg = 0.5 * (rL + rR);
+    const double cs_avg = 0.5 * (csL + csR);
+    const double K = uL - uR;
+    const double p_mid = 0.5 * (pL + pR + K * rho_avg * cs_avg);
+    const double g_fac = (gamma + 1.0) / (2.0 * gamma);
+    const bool lo_L = p_mid <= pL, lo_R = p_mid <= pR;
+    const double cLs = lo_L ? 1.0 : __builtin_sqrt(1.0 + g_fac * (p_mid / pL - 1.0));
+    const double cRs = lo_R ? 1.0 : __builtin_sqrt(1.0 + g_fac * (p_mid / pR - 1.0));
+    const double aL = uL - csL * cLs;
+    const double aR = uR + csR * cRs;
+    const bool bM_is_aL = aL < -tiny, bP_is_aR = tiny < aR;
+    const double bM = bM_is_aL ? aL : -tiny;
+    const double bP = bP_is_aR ? aR : tiny;
+    const double vL1 = uL - aL, vR1 = uR - aR;
+    const double tL = pL + vL1 * qLm, tR = pR + vR1 * qRm;
+    const double mL = rL * vL1, mR = -(rR * vR1);
+    const double inv_m = mL + mR;
+    const double aM = (tL - tR) / inv_m;
+    const double cp0 = (mL * tR + mR * tL) / inv_m;
+    const bool cp_pass = !(cp0 < 0.0);
+    const double cp = cp_pass ? cp0 : 0.0;
+    const double vL2 = uL - bM, vR2 = uR - bP;
+    const double mfL = rL * vL2, mfR = rR * vR2;
+    double fL[4] = {mfL, mfL * wL[1], mfL * wL[2], qLe * vL2 + pL * uL};
+    double fR[4] = {mfR, mfR * wR[1], mfR * wR[2], qRe * vR2 + pR * uR};
+    fL[IV] += pL;
+    fR[IV] += pR;
+    const bool pos = aM >= 0.0;
+    const double d = pos ? aM - bM : bP - aM;
+    const double sL = pos ? aM / d : 0.0;
+    const double sR = pos ? 0.0 : -aM / d;
+    const double sM = pos ? -bM / d : bP / d;
+    // ---- reverse ----
+    double gL[4] = {0.0, 0.0, 0.0, 0.0}, gR[4] = {0.0, 0.0, 0.0, 0.0};  // cotangents of wL, wR
+    const double smcp = sM * cp;
+    double l_sL = 0.0, l_sR = 0.0;
+    for (int v = 0; v < 4; ++v) {
+        l_sL += lf[v] * fL[v];
+        l_sR += lf[v] * fR[v];
+    }
+    const double l_smcp = lf[IV] + lf[3] * aM;  // flux[IM] += sM*cp; flux[E] += (sM*cp)*aM
+    const double l_sM = l_smcp * cp;
+    const double l_cp = l_smcp * sM;
+    double l_aM = lf[3] * smcp;
+    double l_bM = 0.0, l_bP = 0.0;
+    if (pos) {
+        const double l_d = -(l_sL * sL + l_sM * sM) / d;
+        l_aM += l_sL / d + l_d;
+        l_bM += -l_sM / d - l_d;
+    } else {
+        const double l_d = -(l_sR * sR + l_sM * sM) / d;
+        l_aM += -l_sR / d - l_d;
+        l_bP += l_sM / d + l_d;
+    }
+    // fL, fR
+    auto side_f = [&](const double (&w)[4], double s, double mf, double v2, double qe, double p, double u,
+                      double (&g)[4], double& l_b, double& l_qe_out) {
+        const double l0 = s * lf[0], l1 = s * lf[1], l2 = s * lf[2], l3 = s * lf[3];
+        const double l_mf = l0 + l1 * w[1] + l2 * w[2];
+        g[1] += l1 * mf;
+        g[2] += l2 * mf;
+        g[3] += s * lf[IV];  // f[IM] += p
+        // f[E] = qe*v2 + p*u (v2 here: the flux velocity u - b)
+        l_qe_out = l3 * v2;
+        double l_v2 = l3 * qe;
+        g[3] += l3 * u;
+        g[IV] += l3 * p;
+        // mf = rho * v2
+        g[0] += l_mf * v2;
+        l_v2 += l_mf * w[0];
+        // v2 = u - b
+        g[IV] += l_v2;
+        l_b -= l_v2;
+    };
+    double l_qLe = 0.0, l_qRe = 0.0;
+    if (sL != 0.0) side_f(wL, sL, mfL, vL2, qLe, pL, uL, gL, l_bM, l_qLe);
+    if (sR != 0.0) side_f(wR, sR, mfR, vR2, qRe, pR, uR, gR, l_bP, l_qRe);
+    // cp = max(cp0, 0); cp0 = (mL*tR + mR*tL) / inv_m; aM = (tL - tR) / inv_m
+    double l_mL = 0.0, l_mR = 0.0, l_tL = 0.0, l_tR = 0.0, l_inv = 0.0;
+    if (cp_pass) {
+        const double l_n = l_cp / inv_m;
+        l_inv -= l_cp * cp0 / inv_m;
+        l_mL += l_n * tR;
+        l_tR += l_n * mL;
+        l_mR += l_n * tL;
+        l_tL += l_n * mR;
+    }
+    l_tL += l_aM / inv_m;
+    l_tR -= l_aM / inv_m;
+    l_inv -= l_aM * aM / inv_m;
+    l_mL += l_inv;
+    l_mR += l_inv;
+    // mR = -(rR*vR1), mL = rL*vL1, tX = pX + vX1*qXm
+    double l_vR1 = -l_mR * rR + l_tR * qRm;
+    double l_vL1 = l_mL * rL + l_tL * qLm;
+    gR[0] -= l_mR * vR1;
+    gL[0] += l_mL * vL1;
+    gR[3] += l_tR;
+    gL[3] += l_tL;
+    const double l_qRm = l_tR * vR1, l_qLm = l_tL * vL1;
+    // vX1 = uX - aX
+    gR[IV] += l_vR1;
+    gL[IV] += l_vL1;
+    double l_aR = -l_vR1, l_aL = -l_vL1;
+    // bP = max(tiny, aR), bM = min(-tiny, aL)
+    if (bP_is_aR) l_aR += l_bP;
+    if (bM_is_aL) l_aL += l_bM;
+    // aR = uR + csR*cRs, aL = uL - csL*cLs
+    gR[IV] += l_aR;
+    gL[IV] += l_aL;
+    double l_csR = l_aR * cRs, l_csL = -l_aL * cLs;
+    const double l_cRs = l_aR * csR, l_cLs = -l_aL * csL;
+    double l_pmid = 0.0;
+    if (!lo_R) {
+        const double l_x = l_cRs / (2.0 * cRs) * g_fac;  // d/d(p_mid/pR)
+        l_pmid += l_x / pR;
+        gR[3] -= l_x * (p_mid / pR) / pR;
+    }
+    if (!lo_L) {
+        const double l_x = l_cLs / (2.0 * cLs) * g_fac;
+        l_pmid += l_x / pL;
+        gL[3] -= l_x * (p_mid / pL) / pL;
+    }
+    // p_mid = 0.5*(pL + pR + (K*rho_avg)*cs_avg)
+    const double l_h = 0.5 * l_pmid;
+    gL[3] += l_h;
+    gR[3] += l_h;
+    const double l_K = l_h * rho_avg * cs_avg;
+    const double l_ravg = l_h * K * cs_avg;
+    const double l_cavg = l_h * K * rho_avg;
+    gL[IV] += l_K;
+    gR[IV] -= l_K;
+    gL[0] += 0.5 * l_ravg;
+    gR[0] += 0.5 * l_ravg;
+    l_csL += 0.5 * l_cavg;
+    l_csR += 0.5 * l_cavg;
+    // q: qXm = rX*uX, qXe = pX/gm1 + 0.5*rX*|v|^2
+    gL[0] += l_qLm * uL + l_qLe * 0.5 * v2L;
+    gL[IV] += l_qLm * rL;
+    gL[3] += l_qLe / gm1;
+    gL[1] += l_qLe * rL * wL[1];
+    gL[2] += l_qLe * rL * wL[2];
+    gR[0] += l_qRm * uR + l_qRe * 0.5 * v2R;
+    gR[IV] += l_qRm * rR;
+    gR[3] += l_qRe / gm1;
+    gR[1] += l_qRe * rR * wR[1];
+    gR[2] += l_qRe * rR * wR[2];
+    // cs = sqrt(gamma*p/rho)
+    gL[3] += l_csL * gamma / (2.0 * rL * csL);
+    gL[0] -= l_csL * csL / (2.0 * rL);
+    gR[3] += l_csR * gamma / (2.0 * rR * csR);
+    gR[0] -= l_csR * csR / (2.0 * rR);
+    for (int k = 0; k < 4; ++k) {
+        lwL[k] += gL[k];
+        lwR[k] += gR[k];
+    }
+}
+
 // Any Riemann solver through Dual<8>: J^T lf accumulated into lwL, lwR.
 template <RiemannSolver RS, int Axis>
 __host__ __device__ inline void riemann_vjp2_dual(double gamma, const double (&wL)[4], const double (&wR)[4],

@@ -21,6 +21,8 @@ struct FormF64 {
         double gamma;
         unsigned long long* amax_bits;  // CFL max of c_s + |v| (bit pattern)
         unsigned int* floor_hits;       // unused in this form
+        double* frx = nullptr;          // with tracers: density flux of x faces [j][i] (face i of row j)
+        double* fry = nullptr;          // ... of y faces [j][i] (face j of column i)
     };
 
     __device__ static inline double get(const Planes& a, int idx) { return a.p[idx]; }
@@ -28,6 +30,7 @@ struct FormF64 {
     __device__ static inline bool needs_qn(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 5; }
     __device__ static inline double negate(double v) { return -v; }
     __device__ static inline double from_double(double v) { return v; }
+    __device__ static inline double value(double v) { return v; }
 
     // SymmetricOutflowDiode momentum clamp and energy correction,
     // Boundaries.hpp:96-106.

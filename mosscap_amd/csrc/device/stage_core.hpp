@@ -150,6 +150,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         commit_row(j0 + ORD, qb, qhb);
         recon_y(yl, yr);  // cell j0
         riemann_flux<RS, 1, 2>(eos, yRp, yl, Gyp);
+        if (p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
         if (j0 < j1) fetch_row(j0 + 1 + ORD, qa, qha);
@@ -174,6 +175,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         T yl[NV], yr[NV], Gy[NV], dy[NV];
         recon_y(yl, yr);
         riemann_flux<RS, 1, 2>(eos, yRp, yl, Gy);
+        if (p.fry != nullptr && out_lane) p.fry[(j + 1) * xc + c] = Gy[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             dy[v] = Gyp[v] - Gy[v];
@@ -195,6 +197,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
         riemann_flux<RS, 0, 2>(eos, xrm, xl, Fx);
+        if (p.frx != nullptr && lane >= 1 && c >= g.ng && c <= g.ng + g.nx) p.frx[j * xc + c] = Fx[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 

@@ -42,7 +42,6 @@ void set_prim_2d(State& s, double gamma, int j, int i, double rho, double vx, do
 
 void require_2d(const Simulation& sim, const char* name) {
     if (sim.num_dim != 2) throw std::runtime_error(std::string(name) + " only handles 2d problems");
-    if (sim.state.num_tracers != 0) throw std::runtime_error(std::string(name) + " sets no tracer fields");
 }
 
 // ShockTube2d.cpp:10-49: (rho, p) = (1, 1) left of x = 0.5, (0.125, 0.1) right.
@@ -62,7 +61,6 @@ void config_shock_tube_2d(Simulation& sim, const Config& config) {
 // problem.right elsewhere; defaults are Sod's states at x = 0.5.
 void config_shock_tube_1d(Simulation& sim, const Config& config) {
     if (sim.num_dim != 1) throw std::runtime_error("shock_tube_1d only handles 1d problems");
-    if (sim.state.num_tracers != 0) throw std::runtime_error("shock_tube_1d sets no tracer fields");
     auto state_of = [&](const char* key, double r, double v, double p) {
         double w[3] = {r, v, p};
         const YNode* n = find_node(config.get(), std::string("problem.") + key);
@@ -93,7 +91,8 @@ void config_shock_tube_1d(Simulation& sim, const Config& config) {
 // Kelvin-Helmholtz shear band on [0,1]^2: p = 2.5; |y - 0.5| < 0.25 has
 // rho = 2, vx = +0.5, elsewhere rho = 1, vx = -0.5; vx and vy get
 // U(-amp, amp) from the per-cell counters 2*(j*xc + i) and 2*(j*xc + i) + 1
-// of splitmix64(seed + counter).
+// of splitmix64(seed + counter). Tracer fields are a dye of mass fraction 1
+// inside the band. Other problems leave tracer densities at zero.
 void config_kh_band(Simulation& sim, const Config& config) {
     require_2d(sim, "kh_band");
     const uint64_t seed = (uint64_t)get_or<long long>(config, "problem.seed", 20261202LL);
@@ -107,6 +106,8 @@ void config_kh_band(Simulation& sim, const Config& config) {
             const double vx = (band ? 0.5 : -0.5) + amp * (2.0 * uniform01(seed, 2 * idx) - 1.0);
             const double vy = amp * (2.0 * uniform01(seed, 2 * idx + 1) - 1.0);
             set_prim_2d(s, sim.eos.gamma, j, i, band ? 2.0 : 1.0, vx, vy, 2.5);
+            for (int k = 0; k < s.num_tracers; ++k)  // passive dye marking the band
+                s.q(4 + k, 0, j, i) = (band ? 2.0 : 1.0) * (band ? 1.0 : 0.0);
         }
     }
 }

@@ -1,19 +1,28 @@
-// Discrete adjoint of the 2-D stage operator L(Q) = (dt/dx) * (flux
-// divergence of ghost-filled Q), FP64 (-ffp-contract=off). Given a cotangent
-// mu on the interior, computes J_L(Q)^T mu on the interior in five passes:
+// Discrete adjoint of one 2-D stage, FP64 (-ffp-contract=off). A stage maps
+// (Q_s, Q^n) to a_qn*Q^n + a_qs*Q_s + a_l*L(Q_s) with L(Q) = (dt/dx) * (flux
+// divergence of ghost-filled Q). Given the cotangent mu of its output, the
+// stage's reverse gives a_qn*mu to Q^n and a_qs*mu + a_l*J_L(Q_s)^T mu to Q_s.
 //
-//   c2p      W = cons_to_prim(Q) on every cell
-//   faces    per x (y) face: lambda_F = (dt/dx)(mu_right - mu_left), the
-//            Riemann VJP at the reconstructed face states -> cotangents of
-//            RR(left cell) and RL(right cell)
-//   recon    per reconstructed cell and variable: reconstruction VJP -> five
-//            stencil contributions, stored by offset (no atomics)
-//   gather   per cell: sum of the contributions aimed at it, then the
-//            cons_to_prim VJP -> lambda_Q (ghosts included)
-//   fold     transpose of the ghost fill: y ghosts into their sources, then
-//            x ghosts (the fill runs x then y), ghosts cleared.
+// J_L^T mu runs as two line sweeps, one per axis, each a single pass over
+// lines of the grid with one wavefront per 64-cell segment of a line (lanes
+// along the line, every intermediate in LDS):
+//
+//   cons->prim of the segment; reconstruction of every cell; per face the
+//   Riemann VJP of (dt/dx)(mu_right - mu_left) at the reconstructed states
+//   (cotangents of RR of the left cell and RL of the right cell); per cell
+//   the reconstruction VJP into its 2R+1 stencil cells; per cell the sum of
+//   the contributions aimed at it.
+//
+// A segment's outer 2R+1 lanes on each side are halo (their sums are
+// incomplete), so a wave emits 62 - 4R cells. The x sweep writes the
+// primitive cotangents of the interior rows; the y sweep adds its own,
+// applies the cons->prim VJP and the stage combination, and writes every cell
+// of the output (ghost cells get a_l * their cotangent and are folded into
+// their sources by the transposed ghost fill afterwards: y ghosts, then x
+// ghosts, the reverse of the fill).
 #include "../device/adjoint.hpp"
 #include "../device/ghost_dt.hpp"
+#include "../device/wave.hpp"
 #include "../launch_adjoint.hpp"
 
 namespace mh {
@@ -22,120 +31,154 @@ namespace {
 
 constexpr int kB = 256;
 
-__global__ void adj_c2p_kernel(const double* __restrict__ q, double* __restrict__ w, int ncells, double gamma) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncells) return;
-    double qq[4], ww[4];
-    for (int v = 0; v < 4; ++v) qq[v] = q[v * ncells + i];
-    cons_to_prim<2>(gamma, qq, ww);
-    for (int v = 0; v < 4; ++v) w[v * ncells + i] = ww[v];
-}
+struct LineArgs {
+    AdjGrid g;
+    const double* q;
+    const double* mu;
+    double* lamx;
+    double* nxt;
+    double* acc;
+    double a_qn, a_qs, a_l;
+    int32_t nseg;
+};
 
-template <Reconstruction R, SlopeLimiter SL>
-__device__ inline void recon_cell(const double* w, int ncells, int cell, int stride, double (&l)[4], double (&r)[4]) {
-    for (int v = 0; v < 4; ++v) {
-        const double* p = w + v * ncells + cell;
-        auto s = [&](int o) -> double { return p[o * stride]; };
-        reconstruct_stencil<R, SL, double>(s, l[v], r[v]);
-    }
-}
-
-// one thread per face of the given axis: faces f in [ng, ng + n] along the
-// axis, interior along the other axis; face f lies between cells f-1 and f.
-template <Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int Axis>
-__global__ void adj_face_kernel(AdjGrid g, const double* __restrict__ w, const double* __restrict__ mu,
-                                double* __restrict__ lam_rr, double* __restrict__ lam_rl) {
-    const int nface = (Axis == 0 ? g.nx + 1 : g.ny + 1);
-    const int nline = (Axis == 0 ? g.ny : g.nx);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nface * nline) return;
-    const int f = g.ng + t % nface;
-    const int l = g.ng + t / nface;
-    const int i = Axis == 0 ? f : l;
-    const int j = Axis == 0 ? l : f;
-    const int stride = Axis == 0 ? 1 : g.xc;
-    const int cr = j * g.xc + i;
-    const int cl = cr - stride;
-    double wl_l[4], wr_l[4], wl_r[4], wr_r[4];
-    recon_cell<R, SL>(w, g.ncells, cl, stride, wl_l, wr_l);
-    recon_cell<R, SL>(w, g.ncells, cr, stride, wl_r, wr_r);
-    const bool right_in = f < g.ng + (Axis == 0 ? g.nx : g.ny);
-    const bool left_in = f > g.ng;
-    double lf[4];
-    for (int v = 0; v < 4; ++v) {
-        const double mr = right_in ? mu[v * g.ncells + cr] : 0.0;
-        const double ml = left_in ? mu[v * g.ncells + cl] : 0.0;
-        lf[v] = g.dtdx * (mr - ml);
-    }
-    double ll[4] = {0.0, 0.0, 0.0, 0.0}, lr[4] = {0.0, 0.0, 0.0, 0.0};
+template <RiemannSolver RS, int Axis>
+__device__ inline void face_vjp(double gamma, const double (&wl)[4], const double (&wr)[4], const double (&lf)[4],
+                                double (&ll)[4], double (&lr)[4]) {
     if constexpr (RS == RiemannSolver::Rusanov) {
-        rusanov_vjp2<Axis>(g.gamma, wr_l, wl_r, lf, ll, lr);
+        rusanov_vjp2<Axis>(gamma, wl, wr, lf, ll, lr);
+    } else if constexpr (RS == RiemannSolver::Hllc) {
+        hllc_vjp2<Axis>(gamma, wl, wr, lf, ll, lr);
     } else {
-        riemann_vjp2_dual<RS, Axis>(g.gamma, wr_l, wl_r, lf, ll, lr);
-    }
-    for (int v = 0; v < 4; ++v) {
-        lam_rr[v * g.ncells + cl] = ll[v];
-        lam_rl[v * g.ncells + cr] = lr[v];
+        riemann_vjp2_dual<RS, Axis>(gamma, wl, wr, lf, ll, lr);
     }
 }
 
-// one thread per reconstructed cell of the axis ([ng-1, ng+n] along it,
-// interior across): five stencil contributions per variable into gsc[o][v][cell].
-template <Reconstruction R, SlopeLimiter SL, int Axis>
-__global__ void adj_recon_kernel(AdjGrid g, const double* __restrict__ w, const double* __restrict__ lam_rr,
-                                 const double* __restrict__ lam_rl, double* __restrict__ gsc) {
-    const int nrec = (Axis == 0 ? g.nx + 2 : g.ny + 2);
-    const int nline = (Axis == 0 ? g.ny : g.nx);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrec * nline) return;
-    const int c = g.ng - 1 + t % nrec;
-    const int l = g.ng + t / nrec;
-    const int i = Axis == 0 ? c : l;
-    const int j = Axis == 0 ? l : c;
-    const int stride = Axis == 0 ? 1 : g.xc;
-    const int cell = j * g.xc + i;
-    const int nend = g.ng + (Axis == 0 ? g.nx : g.ny);
-    for (int v = 0; v < 4; ++v) {
-        // RR of cell c feeds face c+1 (exists for c <= ng+n-1); RL of c feeds face c (c >= ng)
-        const double lR = (c <= nend - 1) ? lam_rr[v * g.ncells + cell] : 0.0;
-        const double lL = (c >= g.ng) ? lam_rl[v * g.ncells + cell] : 0.0;
-        double s[5], ls[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int o = 0; o < 5; ++o) s[o] = (o >= 2 - stencil_order(R) && o <= 2 + stencil_order(R))
-                                                ? w[v * g.ncells + cell + (o - 2) * stride] : 0.0;
-        if constexpr (R == Reconstruction::Weno5Z) {
-            weno5z_vjp(s, lL, lR, ls);
-        } else {
-            reconstruct_vjp_dual<R, SL>(s, lL, lR, ls);
-        }
-        for (int o = 0; o < 5; ++o) gsc[(o * 4 + v) * g.ncells + cell] = ls[o];
-    }
-}
+// One wavefront per (line, segment). Axis 0: lines are the interior rows,
+// positions run over all columns. Axis 1: lines are all columns (ghost
+// columns only carry the x sweep's cotangents to the epilogue), positions
+// run over all rows.
+template <Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int Axis>
+__global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
+    constexpr int ORD = stencil_order(R);
+    constexpr int HALO = 2 * ORD + 1;
+    constexpr int OUT = kWave - 2 * HALO;
+    __shared__ double sW[4][kWave], sWR[4][kWave], sMu[4][kWave], sLR[4][kWave];
+    __shared__ double sC[2 * ORD + 1][4][kWave];
 
-// per cell: gather the stencil contributions aimed at it from both axes, then
-// the cons_to_prim VJP; lam_q gets J^T mu on every cell (ghosts not yet folded).
-__global__ void adj_gather_kernel(AdjGrid g, const double* __restrict__ q, const double* __restrict__ gx,
-                                  const double* __restrict__ gy, double* __restrict__ lam_q, int order) {
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= g.ncells) return;
-    const int i = cell % g.xc, j = cell / g.xc;
-    double lw[4] = {0.0, 0.0, 0.0, 0.0};
-    const bool row_in = j >= g.ng && j < g.ng + g.ny;
-    const bool col_in = i >= g.ng && i < g.ng + g.nx;
-    for (int o = 2 - order; o <= 2 + order; ++o) {
-        const int d = o - 2;  // this cell is at offset d of the stencil centred on cell - d
-        const int cx = i - d;
-        if (row_in && cx >= g.ng - 1 && cx <= g.ng + g.nx) {
-            for (int v = 0; v < 4; ++v) lw[v] += gx[(o * 4 + v) * g.ncells + cell - d];
-        }
-        const int cy = j - d;
-        if (col_in && cy >= g.ng - 1 && cy <= g.ng + g.ny) {
-            for (int v = 0; v < 4; ++v) lw[v] += gy[(o * 4 + v) * g.ncells + cell - d * g.xc];
+    const AdjGrid& g = a.g;
+    const int t = threadIdx.x;
+    const int seg = blockIdx.x % a.nseg;
+    const int line = blockIdx.x / a.nseg;
+    const int len = Axis == 0 ? g.xc : g.yc;
+    const int nax = Axis == 0 ? g.nx : g.ny;
+    const int p = seg * OUT - HALO + t;  // position along the line
+    const int pc = min(max(p, 0), len - 1);
+    const int li = Axis == 0 ? g.ng + line : line;  // row (axis 0) or column (axis 1)
+    const bool line_in = Axis == 0 ? true : (li >= g.ng && li < g.ng + g.nx);
+    const int cell = Axis == 0 ? li * g.xc + pc : pc * g.xc + li;
+    const int i = Axis == 0 ? pc : li, j = Axis == 0 ? li : pc;
+    const bool cell_in = i >= g.ng && i < g.ng + g.nx && j >= g.ng && j < g.ng + g.ny;
+    const int nc = g.ncells;
+
+    double qq[4], mu[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        qq[v] = a.q[v * nc + cell];
+        mu[v] = cell_in ? a.mu[v * nc + cell] : 0.0;
+    }
+    {
+        double w[4];
+        cons_to_prim<2>(g.gamma, qq, w);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            sW[v][t] = w[v];
+            sMu[v][t] = mu[v];
+            sLR[v][t] = 0.0;
         }
     }
-    double qq[4], lq[4];
-    for (int v = 0; v < 4; ++v) qq[v] = q[v * g.ncells + cell];
-    cons_to_prim_vjp2(g.gamma, qq, lw, lq);
-    for (int v = 0; v < 4; ++v) lam_q[v * g.ncells + cell] = lq[v];
+    wave_lds_sync();
+
+    // reconstruction of the cell at p (cells ng-1 .. ng+n along the axis)
+    const bool rec_ok = line_in && t >= ORD && t <= kWave - 1 - ORD && p >= g.ng - 1 && p <= g.ng + nax;
+    double wl[4] = {0.0, 0.0, 0.0, 0.0}, wr[4] = {0.0, 0.0, 0.0, 0.0};
+    if (rec_ok) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            auto s = [&](int o) -> double { return sW[v][t + o]; };
+            reconstruct_stencil<R, SL, double>(s, wl[v], wr[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sWR[v][t] = wr[v];
+    wave_lds_sync();
+
+    // face at the left edge of p (faces ng .. ng+n): RR of p-1 against RL of p
+    double lL[4] = {0.0, 0.0, 0.0, 0.0};
+    if (line_in && t >= ORD + 1 && t <= kWave - 1 - ORD && p >= g.ng && p <= g.ng + nax) {
+        double wrl[4], lf[4], lrl[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            wrl[v] = sWR[v][t - 1];
+            lf[v] = g.dtdx * (sMu[v][t] - sMu[v][t - 1]);
+        }
+        face_vjp<RS, Axis>(g.gamma, wrl, wl, lf, lrl, lL);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sLR[v][t - 1] = lrl[v];
+    }
+    wave_lds_sync();
+
+    // reconstruction VJP of the cell at p into its stencil
+    const bool vjp_ok = rec_ok && t >= ORD + 1 && t <= kWave - 2 - ORD;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double ls[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (vjp_ok) {
+            double st[5];
+#pragma unroll
+            for (int o = 0; o < 5; ++o) st[o] = (o >= 2 - ORD && o <= 2 + ORD) ? sW[v][t + o - 2] : 0.0;
+            if constexpr (R == Reconstruction::Weno5Z) {
+                weno5z_vjp(st, lL[v], sLR[v][t], ls);
+            } else {
+                reconstruct_vjp_dual<R, SL>(st, lL[v], sLR[v][t], ls);
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < 2 * ORD + 1; ++o) sC[o][v][t] = ls[2 - ORD + o];
+    }
+    wave_lds_sync();
+
+    if (t < HALO || t >= HALO + OUT || p >= len) return;
+    double lw[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double sum = 0.0;
+#pragma unroll
+        for (int o = 0; o < 2 * ORD + 1; ++o) sum += sC[o][v][t - (o - ORD)];
+        lw[v] = sum;
+    }
+    if constexpr (Axis == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a.lamx[v * nc + cell] = lw[v];
+    } else {
+        const bool row_in = j >= g.ng && j < g.ng + g.ny;
+        if (row_in) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) lw[v] += a.lamx[v * nc + cell];
+        }
+        double lq[4];
+        cons_to_prim_vjp2(g.gamma, qq, lw, lq);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = v * nc + cell;
+            if (cell_in) {
+                a.nxt[k] = a.a_qs * mu[v] + a.a_l * lq[v];
+                if (a.a_qn != 0.0) a.acc[k] += a.a_qn * mu[v];
+            } else {
+                a.nxt[k] = a.a_l * lq[v];
+            }
+        }
+    }
 }
 
 // Transpose of one ghost pass: thread per line across the axis, both ends
@@ -179,43 +222,33 @@ __global__ void adj_axpby_kernel(AdjGrid g, double a, const double* __restrict__
 }
 
 template <RiemannSolver RS, Reconstruction R, SlopeLimiter SL>
-hipError_t jt_impl(const AdjGrid& g, const double* q, const double* mu, double* out, const AdjScratch& s,
-                   hipStream_t st) {
-    const int nc = g.ncells;
-    hipLaunchKernelGGL(adj_c2p_kernel, dim3((nc + kB - 1) / kB), dim3(kB), 0, st, q, s.w, nc, g.gamma);
-    const int fx = (g.nx + 1) * g.ny, fy = (g.ny + 1) * g.nx;
-    hipLaunchKernelGGL((adj_face_kernel<R, SL, RS, 0>), dim3((fx + kB - 1) / kB), dim3(kB), 0, st, g, s.w, mu,
-                       s.lam_rr, s.lam_rl);
-    const int rx = (g.nx + 2) * g.ny, ry = (g.ny + 2) * g.nx;
-    hipLaunchKernelGGL((adj_recon_kernel<R, SL, 0>), dim3((rx + kB - 1) / kB), dim3(kB), 0, st, g, s.w, s.lam_rr,
-                       s.lam_rl, s.gx);
-    hipLaunchKernelGGL((adj_face_kernel<R, SL, RS, 1>), dim3((fy + kB - 1) / kB), dim3(kB), 0, st, g, s.w, mu,
-                       s.lam_rr, s.lam_rl);
-    hipLaunchKernelGGL((adj_recon_kernel<R, SL, 1>), dim3((ry + kB - 1) / kB), dim3(kB), 0, st, g, s.w, s.lam_rr,
-                       s.lam_rl, s.gy);
-    hipLaunchKernelGGL(adj_gather_kernel, dim3((nc + kB - 1) / kB), dim3(kB), 0, st, g, q, s.gx, s.gy, out,
-                       stencil_order(R));
-    hipLaunchKernelGGL(adj_fold_kernel<1>, dim3((g.xc + kB - 1) / kB), dim3(kB), 0, st, g, out);
-    hipLaunchKernelGGL(adj_fold_kernel<0>, dim3((g.yc + kB - 1) / kB), dim3(kB), 0, st, g, out);
+hipError_t stage_impl(const AdjGrid& g, const AdjStage& st, hipStream_t stream) {
+    constexpr int OUT = kWave - 2 * (2 * stencil_order(R) + 1);
+    LineArgs a{g, st.q, st.mu, st.lamx, st.nxt, st.acc, st.a_qn, st.a_qs, st.a_l, 0};
+    a.nseg = (g.xc + OUT - 1) / OUT;
+    hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 0>), dim3(a.nseg * g.ny), dim3(kWave), 0, stream, a);
+    a.nseg = (g.yc + OUT - 1) / OUT;
+    hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 1>), dim3(a.nseg * g.xc), dim3(kWave), 0, stream, a);
+    hipLaunchKernelGGL(adj_fold_kernel<1>, dim3((g.xc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
+    hipLaunchKernelGGL(adj_fold_kernel<0>, dim3((g.yc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
     return hipGetLastError();
 }
 
 template <RiemannSolver RS>
-hipError_t jt_rs(Reconstruction r, SlopeLimiter sl, const AdjGrid& g, const double* q, const double* mu, double* out,
-                 const AdjScratch& s, hipStream_t st) {
+hipError_t stage_rs(Reconstruction r, SlopeLimiter sl, const AdjGrid& g, const AdjStage& st, hipStream_t stream) {
     using RC = Reconstruction;
     using SLm = SlopeLimiter;
     switch (r) {
-        case RC::Fog: return jt_impl<RS, RC::Fog, SLm::MonotonizedCentral>(g, q, mu, out, s, st);
-        case RC::Weno5Z: return jt_impl<RS, RC::Weno5Z, SLm::MonotonizedCentral>(g, q, mu, out, s, st);
+        case RC::Fog: return stage_impl<RS, RC::Fog, SLm::MonotonizedCentral>(g, st, stream);
+        case RC::Weno5Z: return stage_impl<RS, RC::Weno5Z, SLm::MonotonizedCentral>(g, st, stream);
         case RC::Muscl:
-            if (sl == SLm::VanLeer) return jt_impl<RS, RC::Muscl, SLm::VanLeer>(g, q, mu, out, s, st);
-            if (sl == SLm::Minmod) return jt_impl<RS, RC::Muscl, SLm::Minmod>(g, q, mu, out, s, st);
-            return jt_impl<RS, RC::Muscl, SLm::MonotonizedCentral>(g, q, mu, out, s, st);
+            if (sl == SLm::VanLeer) return stage_impl<RS, RC::Muscl, SLm::VanLeer>(g, st, stream);
+            if (sl == SLm::Minmod) return stage_impl<RS, RC::Muscl, SLm::Minmod>(g, st, stream);
+            return stage_impl<RS, RC::Muscl, SLm::MonotonizedCentral>(g, st, stream);
         default:
-            if (sl == SLm::VanLeer) return jt_impl<RS, RC::Ppm, SLm::VanLeer>(g, q, mu, out, s, st);
-            if (sl == SLm::Minmod) return jt_impl<RS, RC::Ppm, SLm::Minmod>(g, q, mu, out, s, st);
-            return jt_impl<RS, RC::Ppm, SLm::MonotonizedCentral>(g, q, mu, out, s, st);
+            if (sl == SLm::VanLeer) return stage_impl<RS, RC::Ppm, SLm::VanLeer>(g, st, stream);
+            if (sl == SLm::Minmod) return stage_impl<RS, RC::Ppm, SLm::Minmod>(g, st, stream);
+            return stage_impl<RS, RC::Ppm, SLm::MonotonizedCentral>(g, st, stream);
     }
 }
 
@@ -229,7 +262,7 @@ __global__ void probe_vjp_kernel(int what, int n, const double* in, const double
         if (what == 0) weno5z_vjp(s, cot[2 * t], cot[2 * t + 1], ls);
         else reconstruct_vjp_dual<Reconstruction::Weno5Z, SlopeLimiter::MonotonizedCentral>(s, cot[2 * t], cot[2 * t + 1], ls);
         for (int k = 0; k < 5; ++k) out[5 * t + k] = ls[k];
-    } else {  // Rusanov axis 0 (2: hand, 3: dual) / axis 1 (4: hand, 5: dual): in[8], cot[4] -> out[8]
+    } else {  // Rusanov x (2 hand, 3 dual), Rusanov y (4, 5), HLLC x (6, 7), HLLC y (8, 9): in[8], cot[4] -> out[8]
         double a[4], b[4], lf[4], la[4] = {0, 0, 0, 0}, lb[4] = {0, 0, 0, 0};
         for (int k = 0; k < 4; ++k) {
             a[k] = in[8 * t + k];
@@ -239,7 +272,11 @@ __global__ void probe_vjp_kernel(int what, int n, const double* in, const double
         if (what == 2) rusanov_vjp2<0>(gamma, a, b, lf, la, lb);
         else if (what == 3) riemann_vjp2_dual<RiemannSolver::Rusanov, 0>(gamma, a, b, lf, la, lb);
         else if (what == 4) rusanov_vjp2<1>(gamma, a, b, lf, la, lb);
-        else riemann_vjp2_dual<RiemannSolver::Rusanov, 1>(gamma, a, b, lf, la, lb);
+        else if (what == 5) riemann_vjp2_dual<RiemannSolver::Rusanov, 1>(gamma, a, b, lf, la, lb);
+        else if (what == 6) hllc_vjp2<0>(gamma, a, b, lf, la, lb);
+        else if (what == 7) riemann_vjp2_dual<RiemannSolver::Hllc, 0>(gamma, a, b, lf, la, lb);
+        else if (what == 8) hllc_vjp2<1>(gamma, a, b, lf, la, lb);
+        else riemann_vjp2_dual<RiemannSolver::Hllc, 1>(gamma, a, b, lf, la, lb);
         for (int k = 0; k < 4; ++k) {
             out[8 * t + k] = la[k];
             out[8 * t + 4 + k] = lb[k];
@@ -249,15 +286,14 @@ __global__ void probe_vjp_kernel(int what, int n, const double* in, const double
 
 }  // namespace
 
-hipError_t launch_adjoint_jt(const Schemes& sch, const AdjGrid& g, const double* q, const double* mu, double* out,
-                             const AdjScratch& s, hipStream_t st) {
+hipError_t launch_adjoint_stage(const Schemes& sch, const AdjGrid& g, const AdjStage& st, hipStream_t stream) {
     if (g.nx < g.ng || g.ny < g.ng || g.ng < min_ghost_cells(sch.recon) || g.xc != g.nx + 2 * g.ng ||
-        g.yc != g.ny + 2 * g.ng || g.ncells != g.xc * g.yc)
+        g.yc != g.ny + 2 * g.ng || g.ncells != g.xc * g.yc || !st.q || !st.mu || !st.lamx || !st.nxt || !st.acc)
         return hipErrorInvalidValue;
     switch (sch.solver) {
-        case RiemannSolver::Rusanov: return jt_rs<RiemannSolver::Rusanov>(sch.recon, sch.limiter, g, q, mu, out, s, st);
-        case RiemannSolver::Hll: return jt_rs<RiemannSolver::Hll>(sch.recon, sch.limiter, g, q, mu, out, s, st);
-        default: return jt_rs<RiemannSolver::Hllc>(sch.recon, sch.limiter, g, q, mu, out, s, st);
+        case RiemannSolver::Rusanov: return stage_rs<RiemannSolver::Rusanov>(sch.recon, sch.limiter, g, st, stream);
+        case RiemannSolver::Hll: return stage_rs<RiemannSolver::Hll>(sch.recon, sch.limiter, g, st, stream);
+        default: return stage_rs<RiemannSolver::Hllc>(sch.recon, sch.limiter, g, st, stream);
     }
 }
 
@@ -268,7 +304,7 @@ hipError_t launch_adjoint_axpby(const AdjGrid& g, double a, const double* x, dou
 
 hipError_t launch_probe_vjp(int what, int n, const double* in, const double* cot, double* out, double gamma,
                             hipStream_t st) {
-    if (n < 1 || what < 0 || what > 5) return hipErrorInvalidValue;
+    if (n < 1 || what < 0 || what > 9) return hipErrorInvalidValue;
     hipLaunchKernelGGL(probe_vjp_kernel, dim3((n + kB - 1) / kB), dim3(kB), 0, st, what, n, in, cot, out, gamma);
     return hipGetLastError();
 }

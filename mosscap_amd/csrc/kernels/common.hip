@@ -188,6 +188,20 @@ hipError_t launch_ghost_f32c(int nd, const GhostGrid& g, FormF32c::Planes in, Fo
                              unsigned long long* amax, hipStream_t st) {
     return launch_ghost_impl<FormF32c>(nd, g, in, out, amax, st);
 }
+hipError_t launch_tracer_ghost_f64(const GhostGrid& g, int ntr, FormF64::Planes in, FormF64::Planes out,
+                                   hipStream_t st) {
+    if (ntr < 1 || g.ng < 1 || g.xc < 3 * g.ng || g.yc < 3 * g.ng) return hipErrorInvalidValue;
+    const long long n = 2LL * g.ng * g.yc + 2LL * g.ng * g.xc;
+    hipLaunchKernelGGL(tracer_ghost_kernel<FormF64>, dim3(blocks_for(n)), dim3(kBlock), 0, st, g, ntr, in, out);
+    return hipGetLastError();
+}
+hipError_t launch_tracer_ghost_f32c(const GhostGrid& g, int ntr, FormF32c::Planes in, FormF32c::Planes out,
+                                    hipStream_t st) {
+    if (ntr < 1 || g.ng < 1 || g.xc < 3 * g.ng || g.yc < 3 * g.ng) return hipErrorInvalidValue;
+    const long long n = 2LL * g.ng * g.yc + 2LL * g.ng * g.xc;
+    hipLaunchKernelGGL(tracer_ghost_kernel<FormF32c>, dim3(blocks_for(n)), dim3(kBlock), 0, st, g, ntr, in, out);
+    return hipGetLastError();
+}
 hipError_t launch_cfl_f64(int nd, FormF64::Planes q, int ncells, double gamma, unsigned long long* amax,
                           hipStream_t st) {
     return launch_cfl_impl<FormF64>(nd, q, ncells, gamma, amax, st);

@@ -18,4 +18,9 @@ hipError_t launch_stage1d_f32c(const Schemes& s, int kind, int reduce_dt, int xc
     return launch_stage1d_impl<FormF32c>(s, kind, reduce_dt, xc, ng, p, st);
 }
 
+hipError_t launch_tracer_f32c(const Schemes& s, const TracerGrid& g, const FormF32c::Ptrs& p,
+                              FormF32c::real* wtr, hipStream_t st) {
+    return launch_tracer_impl<FormF32c>(s, g, p, wtr, st);
+}
+
 }  // namespace mh

@@ -17,4 +17,9 @@ hipError_t launch_stage1d_f64(const Schemes& s, int kind, int reduce_dt, int xc,
     return launch_stage1d_impl<FormF64>(s, kind, reduce_dt, xc, ng, p, st);
 }
 
+hipError_t launch_tracer_f64(const Schemes& s, const TracerGrid& g, const FormF64::Ptrs& p,
+                              FormF64::real* wtr, hipStream_t st) {
+    return launch_tracer_impl<FormF64>(s, g, p, wtr, st);
+}
+
 }  // namespace mh

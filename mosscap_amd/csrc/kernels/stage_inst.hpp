@@ -3,6 +3,7 @@
 #pragma once
 
 #include "../launch.hpp"
+#include "../device/tracer.hpp"
 
 // Occupancy target of the fused stage (waves per SIMD the register allocator
 // must allow); each form's translation unit sets its own value.
@@ -110,6 +111,24 @@ hipError_t launch_stage1d_impl(const Schemes& s, int kind, int reduce_dt, int xc
     const Stage1dGrid g{xc, ng, nx, kind, reduce_dt};
     const int segs = (nx + kStripOut - 1) / kStripOut;
     hipLaunchKernelGGL(fn, dim3((unsigned)segs), dim3(kWave), 0, st, g, p);
+    return hipGetLastError();
+}
+
+// Tracer planes of one 2-D stage (after the hydro stage stored the density
+// fluxes): W_t of the stage input, then the interior update.
+template <class Form>
+hipError_t launch_tracer_impl(const Schemes& s, const TracerGrid& g, const typename Form::Ptrs& p,
+                              typename Form::real* wtr, hipStream_t st) {
+    if (g.ntr < 1 || g.nx < 1 || g.ny < 1 || g.ng < min_ghost_cells(s.recon) || g.xc != g.nx + 2 * g.ng ||
+        g.yc != g.ny + 2 * g.ng || !wtr || !p.frx || !p.fry)
+        return hipErrorInvalidValue;
+    const int cells = g.xc * g.yc, inner = g.nx * g.ny;
+    hipLaunchKernelGGL(tracer_prim_kernel<Form>, dim3((cells + 255) / 256), dim3(256), 0, st, g, p.qs, wtr);
+    using Fn = void (*)(TracerGrid, typename Form::Ptrs, const typename Form::real*, const typename Form::real*,
+                        const typename Form::real*);
+    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() -> Fn { return tracer_stage_kernel<Form, R, SL>; };
+    Fn fn = pick_scheme<Form>(s, pick);
+    hipLaunchKernelGGL(fn, dim3((inner + 255) / 256), dim3(256), 0, st, g, p, wtr, p.frx, p.fry);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include "device/form_f32c.hpp"
 #include "device/ghost_dt.hpp"
 #include "device/stage_core.hpp"
+#include "device/tracer.hpp"
 
 namespace mh {
 
@@ -28,6 +29,16 @@ hipError_t launch_stage1d_f64(const Schemes& s, int kind, int reduce_dt, int xc,
                               hipStream_t st);
 hipError_t launch_stage1d_f32c(const Schemes& s, int kind, int reduce_dt, int xc, int ng, const FormF32c::Ptrs& p,
                                hipStream_t st);
+
+// Tracer planes of a 2-D stage (p.frx / p.fry hold the stage's density fluxes).
+hipError_t launch_tracer_f64(const Schemes& s, const TracerGrid& g, const FormF64::Ptrs& p, double* wtr,
+                             hipStream_t st);
+hipError_t launch_tracer_f32c(const Schemes& s, const TracerGrid& g, const FormF32c::Ptrs& p, float* wtr,
+                              hipStream_t st);
+hipError_t launch_tracer_ghost_f64(const GhostGrid& g, int ntr, FormF64::Planes in, FormF64::Planes out,
+                                   hipStream_t st);
+hipError_t launch_tracer_ghost_f32c(const GhostGrid& g, int ntr, FormF32c::Planes in, FormF32c::Planes out,
+                                    hipStream_t st);
 
 hipError_t launch_ghost_f64(int nd, const GhostGrid& g, FormF64::Planes in, FormF64::Planes out,
                             unsigned long long* amax, hipStream_t st);

@@ -14,17 +14,17 @@ struct AdjGrid {
     double dtdx;    // dt / dx of the step (frozen)
 };
 
-struct AdjScratch {
-    double* w;       // primitives [4][cells]
-    double* lam_rr;  // cotangents of the right-face reconstruction [4][cells]
-    double* lam_rl;  // ... of the left-face reconstruction
-    double* gx;      // stencil contributions [5][4][cells], x sweep
-    double* gy;      // ... y sweep
+// One reverse stage of the stage output a_qn*Q^n + a_qs*Q_s + a_l*L(Q_s).
+struct AdjStage {
+    const double* q;   // Q_s, ghost-filled
+    const double* mu;  // cotangent of the stage output (interior; ghost entries zero)
+    double* lamx;      // scratch [4][cells]
+    double* nxt;       // out: a_qs*mu + a_l*J_L(Q_s)^T mu (interior; ghosts folded and zero)
+    double* acc;       // in/out: += a_qn*mu on the interior
+    double a_qn, a_qs, a_l;
 };
 
-// out = J_L(q)^T mu on the interior (ghosts zero); q ghost-filled.
-hipError_t launch_adjoint_jt(const Schemes& sch, const AdjGrid& g, const double* q, const double* mu, double* out,
-                             const AdjScratch& s, hipStream_t st);
+hipError_t launch_adjoint_stage(const Schemes& sch, const AdjGrid& g, const AdjStage& st, hipStream_t stream);
 // y = a*x + b*y on the interior, 0 on ghosts.
 hipError_t launch_adjoint_axpby(const AdjGrid& g, double a, const double* x, double b, double* y, hipStream_t st);
 // unit probes of the hand VJPs against dual-number Jacobians

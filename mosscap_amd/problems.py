@@ -50,8 +50,8 @@ def cell_centres(cfg: Config):
 
 
 def prim_to_cons(cfg: Config, rho, vx, vy, p) -> np.ndarray:
-    """prim_to_cons (Eos.hpp:81-99) on whole fields, same grouping."""
-    q = np.empty(cfg.shape)
+    """prim_to_cons (Eos.hpp:81-99) on whole fields, same grouping; tracer planes zero."""
+    q = np.zeros(cfg.shape)
     q[0, 0] = rho
     q[1, 0] = rho * vx
     v2 = vx * vx
@@ -79,7 +79,8 @@ def kh_band(cfg: Config, seed: int = 20261202, amp: float = 0.01, rho_perturb: n
     """Kelvin-Helmholtz shear band on [0,1]^2: p = 2.5; |y - 0.5| < 0.25 has rho = 2, vx = +0.5,
     elsewhere rho = 1, vx = -0.5; vx and vy get U(-amp, amp) from the per-cell counter
     2*(j*xc + i) (vx) and 2*(j*xc + i) + 1 (vy). `rho_perturb` (interior-shaped) is added
-    to the interior density (the adjoint's control field)."""
+    to the interior density (the adjoint's control field). Tracer fields, if any, are a dye
+    of mass fraction 1 inside the band and 0 outside."""
     x, y = cell_centres(cfg)
     Y = np.broadcast_to(y[:, None], (cfg.yc, cfg.xc))
     band = np.abs(Y - 0.5) < 0.25
@@ -93,7 +94,10 @@ def kh_band(cfg: Config, seed: int = 20261202, amp: float = 0.01, rho_perturb: n
         rho = rho.copy()
         rho[g:g + cfg.ny, g:g + cfg.nx] += rho_perturb
     p = np.full((cfg.yc, cfg.xc), 2.5)
-    return prim_to_cons(cfg, rho, vx, vy, p)
+    q = prim_to_cons(cfg, rho, vx, vy, p)
+    for k in range(cfg.num_tracers):  # passive dye marking the band
+        q[4 + k, 0] = rho * np.where(band, 1.0, 0.0)
+    return q
 
 
 def sedov(cfg: Config, energy: float = 1.0, radius_cells: float = 3.5, p_ambient: float = 1e-5):

@@ -87,7 +87,7 @@ class OracleSim:
         self.cfg = cfg
         c = OrcConfig()
         c.num_dim = cfg.num_dim
-        c.num_tracers = 0
+        c.num_tracers = getattr(cfg, "num_tracers", 0)
         c.xc, c.yc, c.zc, c.ng = cfg.xc, cfg.yc, 1, cfg.ng
         c.dx = cfg.dx
         c.loc[0], c.loc[1], c.loc[2] = cfg.x_start, cfg.y_start, 0.0

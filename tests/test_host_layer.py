@@ -220,6 +220,12 @@ def test_kh_band_ic_matches_python(H):
     np.testing.assert_array_equal(q, problems.kh_band(cfg))
     q2 = H.initial_state(_cfg_yaml("kh_band", 48, 48, 3, extra="  seed: 7\n  amplitude: 0.02\n"))
     np.testing.assert_array_equal(q2, problems.kh_band(cfg, seed=7, amp=0.02))
+    # tracer fields (simulation.n_extra_fields): a dye marking the band
+    text = _cfg_yaml("kh_band", 48, 48, 3).replace("num_dim: 2", "num_dim: 2\n  n_extra_fields: 2")
+    cfg.num_tracers = 2
+    q3 = H.initial_state(text)
+    assert q3.shape == (6, 1, 54, 54)
+    np.testing.assert_array_equal(q3, problems.kh_band(cfg))
 
 
 def test_sedov_ic_matches_python(H):
