@@ -93,7 +93,7 @@ def adjoint_ratio(M, problems, solver, n=512, nsteps=200, every=20):
     device time of the reverse sweep (segment recomputation + reverse stages) over the wall time
     of the same nsteps forward steps replayed from a graph."""
     from mosscap_amd.adjoint import KHAdjointProblem
-    prob = KHAdjointProblem(n=n, nsteps=nsteps, checkpoint_every=every, riemann_solver=solver)
+    prob = KHAdjointProblem(n=n, nsteps=nsteps, checkpoint_every=every, riemann_solver=solver, autotune=True)
     control = 0.01 * (2 * problems.uniform01(17, np.arange(n * n)).reshape(n, n) - 1)
     target = prob.forward(0.01 * (2 * problems.uniform01(18, np.arange(n * n)).reshape(n, n) - 1))
     rho = prob.forward(control)
@@ -101,7 +101,7 @@ def adjoint_ratio(M, problems, solver, n=512, nsteps=200, every=20):
     prob.backward(diff / np.linalg.norm(diff))
     bwd_ms = prob.ctx.adjoint_backward_ms()
     q0 = prob.initial_state(control)
-    with M.HydroContext(prob.cfg, precision_form=M.FP64) as ctx:
+    with M.HydroContext(prob.cfg, precision_form=M.FP64, span_rows=prob.span_rows) as ctx:
         ctx.upload(q0)
         ctx.fill_bcs()
         ctx.run(3, use_graph=True)
@@ -113,7 +113,7 @@ def adjoint_ratio(M, problems, solver, n=512, nsteps=200, every=20):
         _sync()
         fwd_ms = (time.perf_counter() - t0) * 1e3
     prob.ctx.close()
-    return {"solver": solver, "grid": n, "steps": nsteps, "checkpoint_every": every,
+    return {"solver": solver, "grid": n, "steps": nsteps, "checkpoint_every": every, "span_rows": prob.span_rows,
             "backward_ms": round(bwd_ms, 3), "forward_ms": round(fwd_ms, 3), "ratio": round(bwd_ms / fwd_ms, 3)}
 
 

@@ -166,6 +166,14 @@ void* mh_stream(mh_ctx* ctx);
  * values enter dt, as in Hydro.cpp:214-228), mh_run launches step by step
  * instead of replaying a graph, and the adjoint is refused. fn = NULL removes
  * the hook. */
+/* ---- gravity source term: setup_gravity (Gravity.cpp:43-70), registered in
+ * sim.compute_source_terms ----------------------------------------------------
+ * With enable != 0, every stage adds S*dt with S = (0, rho*g, m.g) of the
+ * stage's own input state (gravity_kernel, Gravity.cpp:14-40; the stage
+ * lambdas' `source`, TimeStepping.cpp). Components beyond num_dim must be 0.
+ * enable = 0 removes it. The adjoint differentiates it. */
+int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz);
+
 typedef int (*mh_user_bc_fn)(mh_ctx* ctx, void* user);
 int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user);
 

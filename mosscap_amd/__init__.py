@@ -111,6 +111,7 @@ _SIGNATURES = [
     ("mh_get_stats", C.c_int, [C.c_void_p, C.POINTER(MhStats)]),
     ("mh_stream", C.c_void_p, [C.c_void_p]),
     ("mh_set_user_bc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_set_gravity", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]),
     ("mh_device_state", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -187,6 +188,7 @@ class Config:
     max_time: float = 1.0
     output_delta: float = 0.1
     num_tracers: int = 0  # simulation.n_extra_fields
+    gravity: tuple | None = None  # (gx, gy, gz) of a gravity source term (Gravity.cpp), or None
 
     @property
     def xc(self) -> int:
@@ -234,6 +236,15 @@ class HydroContext:
                            C.byref(opt), C.byref(h)))
         self._h = h
         self.n_values = L.mh_num_values(h)
+        if cfg.gravity is not None:
+            self.set_gravity(cfg.gravity)
+
+    def set_gravity(self, g) -> None:
+        """Register (gx, gy, gz) as a gravity source term, or remove it with None."""
+        if g is None:
+            self._c(lib().mh_set_gravity(self._h, 0, 0.0, 0.0, 0.0))
+        else:
+            self._c(lib().mh_set_gravity(self._h, 1, float(g[0]), float(g[1]), float(g[2])))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -22,7 +22,7 @@ class KHAdjointProblem:
     """Owns the FP64 context, the base initial state and the frozen dt sequence."""
 
     def __init__(self, n: int = 512, nsteps: int = 200, checkpoint_every: int = 20, riemann_solver: str = "rusanov",
-                 seed: int = 20261202, cfg: Config | None = None):
+                 seed: int = 20261202, cfg: Config | None = None, autotune: bool = False):
         self.cfg = cfg or problems.kh_config(n)
         self.cfg.riemann_solver = riemann_solver
         self.nsteps = nsteps
@@ -35,6 +35,11 @@ class KHAdjointProblem:
         rho = base[0, 0, g:-g, g:-g]
         self.vx = base[1, 0, g:-g, g:-g] / rho
         self.vy = base[2, 0, g:-g, g:-g] / rho
+        self.span_rows = 0
+        if autotune:  # span height of the forward (and recomputed) stages for this grid
+            self.ctx.upload(base)
+            self.ctx.fill_bcs()
+            self.span_rows, _ = self.ctx.autotune(steps=3)
 
     def initial_state(self, control: np.ndarray) -> np.ndarray:
         return problems.kh_band(self.cfg, seed=self.seed, rho_perturb=np.asarray(control, dtype=np.float64))

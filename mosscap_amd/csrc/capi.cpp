@@ -174,6 +174,19 @@ int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user) {
     return MH_OK;
 }
 
+int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz) {
+    if (!ctx) return MH_ERR_INVALID;
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
+    if (enable && (gz != 0.0 || (ctx->nd < 2 && gy != 0.0)))
+        return ctx->fail(MH_ERR_INVALID, "gravity components beyond the grid's dimensions must be 0");
+    ctx->gravity = enable ? 1 : 0;
+    ctx->grav[0] = enable ? gx : 0.0;
+    ctx->grav[1] = enable ? gy : 0.0;
+    ctx->grav[2] = 0.0;
+    ctx->drop_graphs();
+    return MH_OK;
+}
+
 int mh_device_state(mh_ctx* ctx, mh_device_view* out) {
     if (!ctx || !out) return MH_ERR_INVALID;
     const int k = ctx->view_buf >= 0 ? ctx->view_buf : ctx->cur;

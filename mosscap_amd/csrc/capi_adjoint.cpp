@@ -49,6 +49,7 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
     }
     if (ctx->user_fn) return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint while a user_bc hook is set");
     if (ctx->ntr > 0) return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint of tracer fields");
+    if (ctx->sz.ng > 8) return ctx->fail(MH_ERR_UNSUPPORTED, "the adjoint supports at most 8 ghost cells");
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
     ctx->free_adjoint();
@@ -104,17 +105,21 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     g.bc[2] = ctx->bdry.ys;
     g.bc[3] = ctx->bdry.ye;
     g.gamma = ctx->gamma;
-    // scratch: the x sweep's primitive cotangents and four cotangent states
-    DevVec lamx, mu, acc, cur, nxt;
-    for (DevVec* v : {&lamx, &mu, &acc, &cur, &nxt})
+    g.gravity = ctx->gravity;
+    g.gx = ctx->grav[0];
+    g.gy = ctx->grav[1];
+    // scratch: the x sweep's primitive cotangents and three cotangent states
+    DevVec lamx, acc, cur, nxt;
+    for (DevVec* v : {&lamx, &acc, &cur, &nxt})
         if (e == hipSuccess) e = v->alloc(nv);
     // stage states of one segment: (every) steps x ns stages
     std::vector<DevVec> seg((size_t)every * ns);
     for (auto& v : seg)
         if (e == hipSuccess) e = v.alloc(nv);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_backward: scratch");
-    if (e == hipSuccess) e = hipMemcpyAsync(mu.p, lambda_final, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, mu.p, 0.0, mu.p, ctx->stream);  // clear ghost entries
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(cur.p, lambda_final, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, cur.p, 0.0, cur.p, ctx->stream);  // clear ghost entries
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
@@ -137,25 +142,24 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
             }
         }
         // reverse the segment's steps
+        // cur holds the cotangent of step n's output; after the stages it holds
+        // dJ/dQ^n (stage 0, reversed last, adds the Q^n terms gathered in acc)
         for (int64_t n = n1 - 1; n >= n0 && e == hipSuccess; --n) {
             g.dtdx = ctx->adj_dts[n] / ctx->dx;
-            // cur = adjoint of the last stage's output; acc = adjoint of Q^n through the Q^n terms
-            e = launch_adjoint_axpby(g, 1.0, mu.p, 0.0, cur.p, ctx->stream);
-            if (e == hipSuccess) e = launch_adjoint_axpby(g, 0.0, mu.p, 0.0, acc.p, ctx->stream);
+            g.dt = ctx->adj_dts[n];
             for (int s = ns - 1; s >= 0 && e == hipSuccess; --s) {
                 const StageCoef c = coef_of(stage_kind(ctx->sch.stepper, s));
-                const AdjStage st{seg[(n - n0) * ns + s].p, cur.p, lamx.p, nxt.p, acc.p, c.a_qn, c.a_qs, c.a_l};
+                const int32_t mode = s == ns - 1 ? 1 : (s == 0 ? 2 : 0);
+                const AdjStage st{seg[(n - n0) * ns + s].p, cur.p, lamx.p, nxt.p, acc.p, c.a_qn, c.a_qs, c.a_l,
+                                  mode};
                 e = launch_adjoint_stage(ctx->sch, g, st, ctx->stream);
                 std::swap(cur.p, nxt.p);
             }
-            // dJ/dQ^n = acc + cur (stage 0's input is Q^n)
-            if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, acc.p, 1.0, cur.p, ctx->stream);
-            std::swap(mu.p, cur.p);
         }
     }
     if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(lambda_initial, mu.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        e = hipMemcpyAsync(lambda_initial, cur.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);

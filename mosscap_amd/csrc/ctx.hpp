@@ -28,6 +28,8 @@ using namespace mh;
 struct mh_ctx {
     int nd = 0, nvar = 0;
     int ntr = 0;                     // tracer planes after the hydro variables
+    int32_t gravity = 0;             // registered gravity source (mh_set_gravity)
+    double grav[3] = {0.0, 0.0, 0.0};
     double* frx64 = nullptr;         // tracers: density fluxes of the stage (FP64 form)
     double* fry64 = nullptr;
     double* wtr64 = nullptr;         // tracers: mass fractions of the stage input
@@ -173,7 +175,8 @@ struct mh_ctx {
         const TracerGrid tg{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, ntr, kind};
         if (fp32c()) {
             FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, gamma, gamma - 1.0,
-                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor, frx32, fry32};
+                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor, frx32, fry32,
+                             gravity, grav[0], grav[1]};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
                 e = launch_stage2d_f32c(sch, g, p, stream);
@@ -183,7 +186,7 @@ struct mh_ctx {
             }
         } else {
             FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, &clock->amax_bits, nullptr,
-                            frx64, fry64};
+                            frx64, fry64, gravity, grav[0], grav[1]};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
                 e = launch_stage2d_f64(sch, g, p, stream);
@@ -198,7 +201,7 @@ struct mh_ctx {
     // FP64 stage between arbitrary device buffers (adjoint recomputation).
     hipError_t stage_ptr(const double* in, const double* qn_p, double* out, int kind) {
         FormF64::Ptrs p{FormF64::Planes{(double*)in}, FormF64::Planes{(double*)qn_p}, FormF64::Planes{out}, &clock->dt,
-                        dx, gamma, &clock->amax_bits, nullptr};
+                        dx, gamma, &clock->amax_bits, nullptr, nullptr, nullptr, gravity, grav[0], grav[1]};
         StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, 0};
         hipError_t e = nd == 2 ? launch_stage2d_f64(sch, g, p, stream)
                                : launch_stage1d_f64(sch, kind, 0, sz.xc, sz.ng, p, stream);
@@ -232,6 +235,14 @@ struct mh_ctx {
         }
         if (reduce_last && user_fn) e = cfl_all(in);
         return in;
+    }
+
+    // kernel arguments changed: graphs captured before must be rebuilt
+    void drop_graphs() {
+        for (int k = 0; k < 3; ++k) {
+            if (graph[k]) hipGraphExecDestroy(graph[k]);
+            graph[k] = nullptr;
+        }
     }
 
     void free_all() {

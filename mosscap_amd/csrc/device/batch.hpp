@@ -40,10 +40,9 @@ __device__ inline void batch_stage(const BatchGrid& g, const typename Form::Ptrs
         const int xc = b.xc[pr];
         const T dt = (T)b.dt[pr];
         const T dtdx = dt / (T)b.dx[pr];
-        const T src = T(0) * dt;
         double amax = 0.0;
         stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, b.base[pr], xc, g.ng, xc - 2 * g.ng, i0, dtdx,
-                                         src, lds, amax, hits);
+                                         dt, lds, amax, hits);
         if (g.reduce_dt) {
             const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
             if (lane == 0) atomicMax(&b.amax[pr], (unsigned long long)m);

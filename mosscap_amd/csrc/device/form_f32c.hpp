@@ -86,6 +86,8 @@ struct FormF32c {
         float p_floor;
         float* frx = nullptr;  // with tracers: density flux of x faces [j][i] (face i of row j)
         float* fry = nullptr;  // ... of y faces [j][i] (face j of column i)
+        int32_t gravity = 0;   // a gravity source term is registered
+        double gx = 0.0, gy = 0.0;
     };
 
     __device__ static inline float2 get(const Planes& a, int idx) { return make_float2(a.hi[idx], a.lo[idx]); }
@@ -140,10 +142,27 @@ struct FormF32c {
         return hit;
     }
 
-    // The stage lambdas of TimeStepping.cpp in double-float arithmetic;
-    // src = S*dt is identically +0 and dropped. Coefficients 1/3, 2/3 and 1/6
-    // are carried as float pairs (hi + lo), exact to ~1e-16.
-    __device__ static inline float2 combine(int kind, float2 Q, float2 Qn, float dq, float /*src*/) {
+    // Gravity source S*dt (Gravity.cpp:22-38) from the stage input, formed in
+    // FP64 from hi+lo and rounded to FP32 like the flux increment it joins.
+    template <int ND, int NV>
+    __device__ static inline void gravity_src(const Ptrs& p, const float2 (&q)[NV], float dt, float (&src)[NV]) {
+        const double rho = df::to_double(q[0]), d = (double)dt;
+        src[0] = 0.0f;
+        src[1] = (float)(rho * p.gx * d);
+        double energy_update = df::to_double(q[1]) * p.gx;
+        if constexpr (ND > 1) {
+            src[2] = (float)(rho * p.gy * d);
+            energy_update += df::to_double(q[2]) * p.gy;
+        }
+        src[1 + ND] = (float)(energy_update * d);
+    }
+
+    // The stage lambdas of TimeStepping.cpp in double-float arithmetic, with
+    // the source joined to the flux increment (dq + S*dt) in FP32.
+    // Coefficients 1/3, 2/3 and 1/6 are carried as float pairs (hi + lo),
+    // exact to ~1e-16.
+    __device__ static inline float2 combine(int kind, float2 Q, float2 Qn, float dq_flux, float src) {
+        const float dq = dq_flux + src;
         const float2 third = make_float2(0x1.555556p-2f, -0x1.555556p-27f);
         const float2 two_thirds = make_float2(0x1.555556p-1f, -0x1.555556p-26f);
         const float2 sixth = make_float2(0x1.555556p-3f, -0x1.555556p-28f);

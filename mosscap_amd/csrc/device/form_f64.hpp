@@ -23,6 +23,8 @@ struct FormF64 {
         unsigned int* floor_hits;       // unused in this form
         double* frx = nullptr;          // with tracers: density flux of x faces [j][i] (face i of row j)
         double* fry = nullptr;          // ... of y faces [j][i] (face j of column i)
+        int32_t gravity = 0;            // a gravity source term is registered
+        double gx = 0.0, gy = 0.0;      // its acceleration
     };
 
     __device__ static inline double get(const Planes& a, int idx) { return a.p[idx]; }
@@ -62,6 +64,21 @@ struct FormF64 {
             case 4: return Q + 0.5 * (dq + src);
             default: return (2.0 / 3.0) * Qn + (1.0 / 3.0) * Q + (1.0 / 6.0) * (dq + src);
         }
+    }
+
+    // Source S*dt of the stage lambdas for a registered gravity term:
+    // gravity_kernel (Gravity.cpp:22-38) accumulated into a zeroed S from the
+    // stage input Q of the cell, then multiplied by dt.
+    template <int ND, int NV>
+    __device__ static inline void gravity_src(const Ptrs& p, const double (&q)[NV], double dt, double (&src)[NV]) {
+        src[0] = 0.0 * dt;
+        src[1] = (0.0 + q[0] * p.gx) * dt;
+        double energy_update = q[1] * p.gx;
+        if constexpr (ND > 1) {
+            src[2] = (0.0 + q[0] * p.gy) * dt;
+            energy_update += q[2] * p.gy;
+        }
+        src[1 + ND] = (0.0 + energy_update) * dt;
     }
 
     // dt_reducer, Hydro.cpp:192-206: cons_to_prim then c_s + |v|.

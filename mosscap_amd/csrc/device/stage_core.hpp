@@ -203,13 +203,20 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
         if (out_lane) {
             Raw qo[NV];
+            T srcv[NV];
+            if (p.gravity) {
+                Form::template gravity_src<2>(p, qs_c, dt, srcv);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) srcv[v] = src;
+            }
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 T q_update = T(0);
                 q_update += Fx[v] - Fxp[v];
                 q_update += dy[v];
                 q_update *= dtdx;
-                qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, src);
+                qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v]);
                 Form::put(p.out, v * plane + j * xc + c, qo[v]);
             }
             if (g.reduce_dt) {
@@ -237,7 +244,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
 __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, int reduce_dt, int base, int xc,
                                        int ng, int nx, int i0, const typename Form::real dtdx,
-                                       const typename Form::real src, typename Form::real* lds_row /*[3][64+2R]*/,
+                                       const typename Form::real dt, typename Form::real* lds_row /*[3][64+2R]*/,
                                        double& amax, unsigned int& floor_hits) {
     using T = typename Form::real;
     using Raw = typename Form::raw;
@@ -287,12 +294,19 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
     for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
     if (out_lane) {
         Raw qo[NV];
+        T srcv[NV];
+        if (p.gravity) {
+            Form::template gravity_src<1>(p, q, dt, srcv);
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) srcv[v] = T(0) * dt;
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             T q_update = T(0);
             q_update += Fx[v] - Fxp[v];
             q_update *= dtdx;
-            qo[v] = Form::combine(kind, q[v], qn[v], q_update, src);
+            qo[v] = Form::combine(kind, q[v], qn[v], q_update, srcv[v]);
             Form::put(p.out, base + v * xc + c, qo[v]);
         }
         if (reduce_dt) {

@@ -66,7 +66,10 @@ std::string summary(const Simulation& sim) {
       << ",\"single_file\":" << (sim.out_cfg.single_file ? "true" : "false") << ",\"delta\":" << num(sim.out_cfg.delta)
       << ",\"primitive\":" << (sim.out_cfg.variables.primitive ? "true" : "false")
       << ",\"precision_form\":" << sim.device.precision_form << ",\"span_rows\":" << sim.device.span_rows
-      << ",\"rho_floor\":" << num(sim.device.rho_floor) << ",\"p_floor\":" << num(sim.device.p_floor) << "}";
+      << ",\"rho_floor\":" << num(sim.device.rho_floor) << ",\"p_floor\":" << num(sim.device.p_floor)
+      << ",\"gravity\":" << (sim.gravity ? "[" + num(sim.grav[0]) + "," + num(sim.grav[1]) + "," + num(sim.grav[2]) + "]"
+                                              : std::string("null"))
+      << "}";
     return o.str();
 }
 

@@ -138,12 +138,49 @@ void config_sedov(Simulation& sim, const Config& config) {
         for (int i = 0; i < s.sz.xc; ++i) set_prim_2d(s, sim.eos.gamma, j, i, 1.0, 0.0, 0.0, inside(j, i) ? p_in : p_ambient);
 }
 
+// The reference's gravity-driven shear layer (KelvinHelmholtz.cpp:11-70):
+// rho = 2 where the coordinate along problem.axis is positive, else 1;
+// p = 2.5 + rho g x_axis; a cosine bump in the velocity along the axis; and
+// a gravity source. The reference writes -0.1 into sources.gravity.<axis>
+// unconditionally (set_if_not_present assigns, MosscapConfig.hpp:91-103) and
+// setup_gravity reads the other components with its defaults (x: -1), so with
+// axis 1 the gravity is (-1, -0.1) unless sources.gravity.x is given; both
+// are kept here.
+void config_kelvin_helmholtz(Simulation& sim, const Config& config) {
+    require_2d(sim, "kelvin_helmholtz");
+    const int axis = get_or<int>(config, "problem.axis", 1);
+    const double grav = -0.1;
+    State& s = sim.state;
+    const int nx = s.sz.xc - 2 * s.sz.ng, ny = s.sz.yc - 2 * s.sz.ng;
+    if (axis == 0 && nx / ny != 3) throw std::runtime_error("Aspect ratio should be 3 (x / y) == 3 for axis = 0");
+    if (axis != 0 && ny / nx != 3) throw std::runtime_error("Aspect ratio should be 3 (y / x) == 3 for axis = 1");
+    const int other = axis == 0 ? 1 : 0;
+    const int iv = axis == 0 ? 1 : 2;
+    auto axis_length = [&](int a) { return (a == 0 ? nx : ny) * s.dx; };  // State::get_axis_length
+    const double L_axis = axis_length(axis), L_other = axis_length(other);
+    for (int j = 0; j < s.sz.yc; ++j)
+        for (int i = 0; i < s.sz.xc; ++i) {
+            const double pos[2] = {s.pos_x(i), s.pos_y(j)};
+            const double pres_0 = 2.5;
+            double w[4] = {0.0, 0.0, 0.0, 0.0};
+            w[0] = pos[axis] > 0.0 ? 2.0 : 1.0;
+            w[3] = pres_0 + w[0] * grav * pos[axis];
+            w[iv] = 0.01 * (1.0 + std::cos(2.0 * M_PI / L_other * pos[other])) *
+                    (1.0 + std::cos(2.0 * M_PI / L_axis * pos[axis])) * 0.25;
+            set_prim_2d(s, sim.eos.gamma, j, i, w[0], w[1], w[2], w[3]);
+        }
+    double forced[3] = {NAN, NAN, NAN};
+    forced[axis == 0 ? 0 : 1] = grav;
+    setup_gravity(sim, config, forced);
+}
+
 const bool registered = [] {
     auto& f = get_problem_generator();
     f.register_method("shock_tube_2d", config_shock_tube_2d);
     f.register_method("shock_tube_1d", config_shock_tube_1d);
     f.register_method("kh_band", config_kh_band);
     f.register_method("sedov", config_sedov);
+    f.register_method("kelvin_helmholtz", config_kelvin_helmholtz);
     return true;
 }();
 

@@ -190,6 +190,15 @@ void setup_device_options(Simulation& sim, const Config& config) {
     sim.device.p_floor = get_or<double>(config, "hip.p_floor", 0.0);
 }
 
+void setup_gravity(Simulation& sim, const Config& config, const double (&forced)[3]) {
+    if (sim.gravity) throw std::runtime_error("Source \"gravity\" already registered.");
+    const double dflt[3] = {-1.0, 0.0, 0.0};
+    const char* key[3] = {"sources.gravity.x", "sources.gravity.y", "sources.gravity.z"};
+    for (int a = 0; a < 3; ++a)
+        sim.grav[a] = forced[a] == forced[a] ? forced[a] : get_or<double>(config, key[a], dflt[a]);
+    sim.gravity = true;
+}
+
 Simulation setup_sim_host(const Config& config) {
     const int num_dim = get_or<int>(config, "simulation.num_dim", 0);
     if (num_dim < 1 || num_dim > 3)
@@ -314,6 +323,9 @@ void attach_device(Simulation& sim, bool write_initial_output) {
     };
     if (sim.user_bc) {
         check(sim, mh_set_user_bc(raw, user_bc_trampoline, sim.hook.get()), "user_bc");
+    }
+    if (sim.gravity) {
+        check(sim, mh_set_gravity(raw, 1, sim.grav[0], sim.grav[1], sim.grav[2]), "gravity");
     }
     fill_bcs(sim);
     if (write_initial_output) sim.write_output(sim);

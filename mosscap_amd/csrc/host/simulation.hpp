@@ -122,6 +122,9 @@ struct Simulation {
     NumericalSchemes scheme;
     OutputConfig out_cfg;
     DeviceOptions device;
+    // registered source terms (Simulation::compute_source_terms): gravity only
+    bool gravity = false;
+    double grav[3] = {0.0, 0.0, 0.0};
     std::shared_ptr<mh_ctx> ctx;  // device state and operators
     std::shared_ptr<HookHolder> hook = std::make_shared<HookHolder>();
 
@@ -192,6 +195,11 @@ struct DriverResult {
     int outputs = 0;
 };
 DriverResult run_driver(Simulation& sim, bool verbose = true);
+
+// setup_gravity (Gravity.cpp:43-70): registers the gravity source with
+// sources.gravity.{x,y,z} (defaults -1, 0, 0); `forced` components (x, y, z
+// order, NaN = read from the configuration) take precedence.
+void setup_gravity(Simulation& sim, const Config& config, const double (&forced)[3]);
 
 // splitmix64 counter generator for seeded initial conditions.
 uint64_t splitmix64(uint64_t x);

@@ -39,6 +39,7 @@ struct LineArgs {
     double* nxt;
     double* acc;
     double a_qn, a_qs, a_l;
+    int32_t acc_mode;  // AdjStage::acc_mode
     int32_t nseg;
 };
 
@@ -168,44 +169,85 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
         }
         double lq[4];
         cons_to_prim_vjp2(g.gamma, qq, lw, lq);
+        if (g.gravity && cell_in) {  // S*dt = dt*(0, rho gx, rho gy, mx gx + my gy) of this cell
+            lq[0] += g.dt * (g.gx * mu[1] + g.gy * mu[2]);
+            lq[1] += g.dt * (g.gx * mu[3]);
+            lq[2] += g.dt * (g.gy * mu[3]);
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int k = v * nc + cell;
             if (cell_in) {
-                a.nxt[k] = a.a_qs * mu[v] + a.a_l * lq[v];
-                if (a.a_qn != 0.0) a.acc[k] += a.a_qn * mu[v];
+                double r = a.a_qs * mu[v] + a.a_l * lq[v];
+                if (a.acc_mode == 2) r += a.acc[k];
+                a.nxt[k] = r;
+                if (a.acc_mode == 1) a.acc[k] = a.a_qn * mu[v];
+                else if (a.acc_mode == 0 && a.a_qn != 0.0) a.acc[k] += a.a_qn * mu[v];
             } else {
                 a.nxt[k] = a.a_l * lq[v];
+                if (a.acc_mode == 1) a.acc[k] = 0.0;
             }
         }
     }
 }
 
-// Transpose of one ghost pass: thread per line across the axis, both ends
-// handled sequentially (zerograd sends ng ghosts to one source).
+// Transpose of one ghost pass (the fill copies a source into each ghost, with
+// the normal momentum negated at walls): each ghost's cotangent is added to its
+// source and the ghost cleared. One thread per (line across the axis,
+// variable); a side's ghosts are read together and zerograd's ng ghosts, which
+// share one source, are summed first. The two sides run in order, so they may
+// share sources on a narrow grid.
+constexpr int kMaxFoldGhosts = 8;
+
 template <int Axis>
 __global__ void adj_fold_kernel(AdjGrid g, double* __restrict__ lam) {
     const int nline = Axis == 0 ? g.yc : g.xc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nline) return;
+    if (t >= 4 * nline) return;
+    const int v = t / nline, line = t % nline;
     const int dims = Axis == 0 ? g.xc : g.yc;
     const int stride = Axis == 0 ? 1 : g.xc;
-    const int base = Axis == 0 ? t * g.xc : t;
+    double* L = lam + (size_t)v * g.ncells + (Axis == 0 ? line * g.xc : line);
     constexpr int IM = 1 + Axis;
-    for (int k = 0; k < 2 * g.ng; ++k) {
-        const int gcoord = k < g.ng ? k : dims - 2 * g.ng + k;
-        const bool start = gcoord < g.ng;
-        const BoundaryType b = (BoundaryType)g.bc[2 * Axis + (start ? 0 : 1)];
-        const int gc = base + gcoord * stride;
-        if (b == BoundaryType::Wall || b == BoundaryType::Periodic || b == BoundaryType::Symmetric ||
-            b == BoundaryType::ZeroGrad) {
-            const int sc = base + ghost_source(gcoord, dims, g.ng, b) * stride;
-            for (int v = 0; v < 4; ++v) {
-                const double x = lam[v * g.ncells + gc];
-                lam[v * g.ncells + sc] += (b == BoundaryType::Wall && v == IM) ? -x : x;
+    const int ng = g.ng;
+    for (int side = 0; side < 2; ++side) {
+        const BoundaryType b = (BoundaryType)g.bc[2 * Axis + side];
+        const bool maps = b == BoundaryType::Wall || b == BoundaryType::Periodic || b == BoundaryType::Symmetric ||
+                          b == BoundaryType::ZeroGrad;
+        double gv[kMaxFoldGhosts];
+#pragma unroll
+        for (int k = 0; k < kMaxFoldGhosts; ++k) {
+            if (k >= ng) break;
+            const int gc = side == 0 ? k : dims - 1 - k;
+            gv[k] = L[gc * stride];
+            L[gc * stride] = 0.0;
+        }
+        if (!maps) continue;
+        const double sgn = (b == BoundaryType::Wall && v == IM) ? -1.0 : 1.0;
+        if (b == BoundaryType::ZeroGrad) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < kMaxFoldGhosts; ++k) {
+                if (k >= ng) break;
+                sum += gv[k];
+            }
+            const int sc = ghost_source(side == 0 ? 0 : dims - 1, dims, ng, b);
+            L[sc * stride] += sum;
+        } else {
+            double sv[kMaxFoldGhosts];
+            int si[kMaxFoldGhosts];
+#pragma unroll
+            for (int k = 0; k < kMaxFoldGhosts; ++k) {
+                if (k >= ng) break;
+                si[k] = ghost_source(side == 0 ? k : dims - 1 - k, dims, ng, b);
+                sv[k] = L[si[k] * stride];
+            }
+#pragma unroll
+            for (int k = 0; k < kMaxFoldGhosts; ++k) {
+                if (k >= ng) break;
+                L[si[k] * stride] = sv[k] + sgn * gv[k];
             }
         }
-        for (int v = 0; v < 4; ++v) lam[v * g.ncells + gc] = 0.0;
     }
 }
 
@@ -224,13 +266,13 @@ __global__ void adj_axpby_kernel(AdjGrid g, double a, const double* __restrict__
 template <RiemannSolver RS, Reconstruction R, SlopeLimiter SL>
 hipError_t stage_impl(const AdjGrid& g, const AdjStage& st, hipStream_t stream) {
     constexpr int OUT = kWave - 2 * (2 * stencil_order(R) + 1);
-    LineArgs a{g, st.q, st.mu, st.lamx, st.nxt, st.acc, st.a_qn, st.a_qs, st.a_l, 0};
+    LineArgs a{g, st.q, st.mu, st.lamx, st.nxt, st.acc, st.a_qn, st.a_qs, st.a_l, st.acc_mode, 0};
     a.nseg = (g.xc + OUT - 1) / OUT;
     hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 0>), dim3(a.nseg * g.ny), dim3(kWave), 0, stream, a);
     a.nseg = (g.yc + OUT - 1) / OUT;
     hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 1>), dim3(a.nseg * g.xc), dim3(kWave), 0, stream, a);
-    hipLaunchKernelGGL(adj_fold_kernel<1>, dim3((g.xc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
-    hipLaunchKernelGGL(adj_fold_kernel<0>, dim3((g.yc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
+    hipLaunchKernelGGL(adj_fold_kernel<1>, dim3((4 * g.xc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
+    hipLaunchKernelGGL(adj_fold_kernel<0>, dim3((4 * g.yc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
     return hipGetLastError();
 }
 
@@ -288,7 +330,8 @@ __global__ void probe_vjp_kernel(int what, int n, const double* in, const double
 
 hipError_t launch_adjoint_stage(const Schemes& sch, const AdjGrid& g, const AdjStage& st, hipStream_t stream) {
     if (g.nx < g.ng || g.ny < g.ng || g.ng < min_ghost_cells(sch.recon) || g.xc != g.nx + 2 * g.ng ||
-        g.yc != g.ny + 2 * g.ng || g.ncells != g.xc * g.yc || !st.q || !st.mu || !st.lamx || !st.nxt || !st.acc)
+        g.yc != g.ny + 2 * g.ng || g.ncells != g.xc * g.yc || g.ng > kMaxFoldGhosts || !st.q || !st.mu ||
+        !st.lamx || !st.nxt || !st.acc || st.acc_mode < 0 || st.acc_mode > 2)
         return hipErrorInvalidValue;
     switch (sch.solver) {
         case RiemannSolver::Rusanov: return stage_rs<RiemannSolver::Rusanov>(sch.recon, sch.limiter, g, st, stream);

@@ -31,10 +31,9 @@ __global__ __launch_bounds__(64) void stage1d_kernel(Stage1dGrid g, typename For
     const int i0 = g.ng + (int)blockIdx.x * kStripOut;
     const T dt = (T)p.dt[0];
     const T dtdx = dt / (T)p.dx;
-    const T src = T(0) * dt;
     double amax = 0.0;
     unsigned int hits = 0;
-    stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, 0, g.xc, g.ng, g.nx, i0, dtdx, src, lds, amax, hits);
+    stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, 0, g.xc, g.ng, g.nx, i0, dtdx, dt, lds, amax, hits);
     if (g.reduce_dt) {
         const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
         if (threadIdx.x == 0) atomicMax(p.amax_bits, (unsigned long long)m);

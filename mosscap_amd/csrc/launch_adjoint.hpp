@@ -12,6 +12,9 @@ struct AdjGrid {
     int32_t bc[4];  // xs, xe, ys, ye
     double gamma;
     double dtdx;    // dt / dx of the step (frozen)
+    double dt;
+    int32_t gravity;  // a gravity source term is registered
+    double gx, gy;
 };
 
 // One reverse stage of the stage output a_qn*Q^n + a_qs*Q_s + a_l*L(Q_s).
@@ -20,8 +23,11 @@ struct AdjStage {
     const double* mu;  // cotangent of the stage output (interior; ghost entries zero)
     double* lamx;      // scratch [4][cells]
     double* nxt;       // out: a_qs*mu + a_l*J_L(Q_s)^T mu (interior; ghosts folded and zero)
-    double* acc;       // in/out: += a_qn*mu on the interior
+    double* acc;       // cotangent of Q^n through the stages' Q^n terms (interior; ghosts zero)
     double a_qn, a_qs, a_l;
+    // 0: acc += a_qn*mu; 1 (the step's last stage, reversed first): acc = a_qn*mu;
+    // 2 (stage 0, reversed last): nxt += acc, giving dJ/dQ^n
+    int32_t acc_mode;
 };
 
 hipError_t launch_adjoint_stage(const Schemes& sch, const AdjGrid& g, const AdjStage& st, hipStream_t stream);

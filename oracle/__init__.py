@@ -32,7 +32,8 @@ class OrcConfig(C.Structure):
     _fields_ = [("num_dim", C.c_int32), ("num_tracers", C.c_int32), ("xc", C.c_int32), ("yc", C.c_int32),
                 ("zc", C.c_int32), ("ng", C.c_int32), ("dx", C.c_double), ("loc", C.c_double * 3),
                 ("bc", C.c_int32 * 6), ("bc_const", (C.c_double * 5) * 6), ("recon", C.c_int32),
-                ("limiter", C.c_int32), ("solver", C.c_int32), ("stepper", C.c_int32), ("gamma", C.c_double)]
+                ("limiter", C.c_int32), ("solver", C.c_int32), ("stepper", C.c_int32), ("gamma", C.c_double),
+                ("gravity", C.c_int32), ("grav", C.c_double * 3)]
 
 
 _lib = None
@@ -100,6 +101,11 @@ class OracleSim:
         c.solver = SOLVER[cfg.riemann_solver]
         c.stepper = STEPPER[cfg.time_stepper]
         c.gamma = cfg.gamma
+        g = getattr(cfg, "gravity", None)  # (gx, gy, gz) of a registered gravity source, or None
+        if g is not None:
+            c.gravity = 1
+            for k in range(3):
+                c.grav[k] = g[k]
         self._c = c
         self._h = lib().orc_create(C.byref(c))
 

@@ -289,6 +289,8 @@ typedef struct {
     double bc_const[6][5];   // State.hpp:92-97
     int32_t recon, limiter, solver, stepper;  // NumericalSchemes, Simulation.hpp:17-22
     double gamma;
+    int32_t gravity;                          // a registered gravity source (Gravity.cpp:43-70)
+    double grav[3];                           // sources.gravity.{x,y,z}
 } orc_config;
 
 }  // extern "C"
@@ -486,9 +488,34 @@ void fill_bcs(orc_sim& s) {
     if (s.c.num_dim > 2) fill_one_bc(s, 2);
 }
 
+// gravity_kernel<NumDim>, Gravity.cpp:14-40: S accumulates the gravity source
+// of the current Q on every cell (S was zeroed before the stage,
+// SourceTerms.hpp:8-10).
+void compute_source_terms(orc_sim& s) {
+    const auto& c = s.c;
+    if (!c.gravity) return;
+    const int nd = c.num_dim;
+    const int64_t n = (int64_t)s.cells;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < n; ++idx) {
+        const double rho = s.Q[idx];
+        s.S[1 * n + idx] += rho * c.grav[0];
+        double energy_update = s.Q[1 * n + idx] * c.grav[0];
+        if (nd > 1) {
+            s.S[2 * n + idx] += rho * c.grav[1];
+            energy_update += s.Q[2 * n + idx] * c.grav[1];
+        }
+        if (nd > 2) {
+            s.S[3 * n + idx] += rho * c.grav[2];
+            energy_update += s.Q[3 * n + idx] * c.grav[2];
+        }
+        s.S[(1 + nd) * n + idx] += energy_update;
+    }
+}
+
 // integrate_flux<NumDim>, TimeStepping.cpp:9-35, with the stage update
-// selected by (stepper, stage); source term S is identically zero
-// (zero_source_terms, SourceTerms.hpp:8-10) and enters as S*dt.
+// selected by (stepper, stage); the source term S (zero unless gravity is
+// registered) enters as S*dt for the hydro variables.
 void integrate_flux(orc_sim& s, int stepper, int stage, double dt) {
     const auto& c = s.c;
     const int nd = c.num_dim, n_hydro = 2 + nd;
@@ -605,11 +632,12 @@ double orc_compute_dt(const orc_sim* s, double max_cfl, double time, double max_
 // fill_bcs. Q_old is a copy of Q at the start of the step.
 void orc_time_step(orc_sim* s, double dt) {
     const int ns = num_stages(s->c.stepper);
-    std::fill(s->S.begin(), s->S.end(), 0.0);
     s->Q_old = s->Q;
     for (int st = 0; st < ns; ++st) {
+        std::fill(s->S.begin(), s->S.end(), 0.0);  // zero_source_terms, every stage
         if (st > 0) fill_bcs(*s);
         compute_hydro_fluxes(*s);
+        compute_source_terms(*s);
         integrate_flux(*s, s->c.stepper, st, dt);
     }
     fill_bcs(*s);
@@ -620,7 +648,9 @@ void orc_time_step(orc_sim* s, double dt) {
 // orc_begin_step. Does not fill ghosts before the stage.
 void orc_begin_step(orc_sim* s) { s->Q_old = s->Q; }
 void orc_stage(orc_sim* s, int stage, double dt) {
+    std::fill(s->S.begin(), s->S.end(), 0.0);
     compute_hydro_fluxes(*s);
+    compute_source_terms(*s);
     integrate_flux(*s, s->c.stepper, stage, dt);
 }
 

@@ -106,7 +106,13 @@ def test_autograd_function_roundtrip():
 
 @pytest.mark.parametrize("solver", ["rusanov", "hllc"])
 def test_full_workload_fd_agreement(solver):
-    """KH 512^2, 200 steps, dt frozen, checkpoint every 20; 16 seeded directions."""
+    """KH 512^2, 200 steps, dt frozen, checkpoint every 20; 16 seeded directions.
+
+    Central differences at eps = 1e-3 carry an O(eps) error of their own: the discrete
+    map has kinks (|beta0 - beta2| in WENO5-Z, the HLLC and min/max switches) that a
+    perturbation of that size crosses in some cells. The deviation from the adjoint
+    shrinks with eps (1e-3: <= 5e-4 of the largest |FD|; 1e-5: ~1e-7), which is the
+    check that the adjoint is the derivative of the map it differentiates."""
     prob = KHAdjointProblem(n=512, nsteps=200, checkpoint_every=20, riemann_solver=solver)
     n = prob.cfg.nx
     control = 0.01 * (2 * problems.uniform01(17, np.arange(n * n)).reshape(n, n) - 1)
@@ -123,15 +129,17 @@ def test_full_workload_fd_agreement(solver):
     t0 = time.perf_counter()
     grad = prob.backward(diff / np.linalg.norm(diff))
     t_bwd = time.perf_counter() - t0
-    eps = 1e-3
-    rel = []
-    fds = []
-    for k in range(16):
-        d = 2 * problems.uniform01(19 + 1000 * k, np.arange(n * n)).reshape(n, n) - 1
-        d /= np.linalg.norm(d)
-        fd = (loss(control + eps * d) - loss(control - eps * d)) / (2 * eps)
-        fds.append(fd)
-        rel.append(abs(float((grad * d).sum()) - fd))
-    rel = np.array(rel) / max(abs(f) for f in fds)
-    print(f"{solver}: max rel FD error {rel.max():.2e}; backward/forward wall {t_bwd / t_fwd:.2f}")
-    assert rel.max() <= 1e-4, rel
+    rel = {}
+    for eps in (1e-3, 1e-5):
+        dev, fds = [], []
+        for k in range(16):
+            d = 2 * problems.uniform01(19 + 1000 * k, np.arange(n * n)).reshape(n, n) - 1
+            d /= np.linalg.norm(d)
+            fd = (loss(control + eps * d) - loss(control - eps * d)) / (2 * eps)
+            fds.append(fd)
+            dev.append(abs(float((grad * d).sum()) - fd))
+        rel[eps] = np.array(dev) / max(abs(f) for f in fds)
+    print(f"{solver}: max rel FD deviation eps=1e-3 {rel[1e-3].max():.2e}, eps=1e-5 {rel[1e-5].max():.2e}; "
+          f"backward/forward wall {t_bwd / t_fwd:.2f}")
+    assert rel[1e-5].max() <= 1e-5, rel[1e-5]
+    assert rel[1e-3].max() <= 1e-3, rel[1e-3]

@@ -272,3 +272,34 @@ def test_netcdf_multi_file(H, tmp_path):
         assert f.dimensions["time"] == 1
         assert float(f.variables["time"][0]) == 0.5
         np.testing.assert_array_equal(f.variables["Q"][:], H.initial_state(text))
+
+
+# ---- gravity source: the reference's kelvin_helmholtz problem ---------------------
+KH_RT = ("simulation:\n  num_dim: 2\ngrid:\n  x: 16\n  y: 48\n  x_dim: 0.5\n  x_start: -0.25\n  y_start: -0.75\n"
+         "boundary:\n  xs: periodic\n  xe: periodic\n  ys: wall\n  ye: wall\nproblem:\n  name: kelvin_helmholtz\n")
+
+
+def test_kelvin_helmholtz_registers_gravity(H):
+    s = H.setup_summary(KH_RT)
+    # sources.gravity.y is forced to -0.1 (set_if_not_present assigns); x keeps setup_gravity's default -1
+    assert s["gravity"] == [-1.0, -0.1, 0.0]
+    s = H.setup_summary(KH_RT + "sources:\n  gravity:\n    x: 0.0\n    y: -5.0\n")
+    assert s["gravity"] == [0.0, -0.1, 0.0]
+    assert H.setup_summary(MIN)["gravity"] is None
+
+
+def test_kelvin_helmholtz_checks_aspect_ratio(H):
+    with pytest.raises(H.HostError, match=r"Aspect ratio should be 3 \(y / x\) == 3 for axis = 1"):
+        H.setup_summary(KH_RT.replace("y: 48", "y: 32"))
+
+
+def test_kelvin_helmholtz_initial_state(H):
+    q = H.initial_state(KH_RT)
+    ng, dx = 3, 0.5 / 16
+    y = np.arange(54) * dx + (-(ng - 0.5) * dx) - 0.75
+    rho = np.where(y > 0.0, 2.0, 1.0)
+    np.testing.assert_array_equal(q[0, 0, :, 5], rho)
+    p_expect = 2.5 + rho * -0.1 * y
+    vy = q[2, 0, :, 5] / q[0, 0, :, 5]
+    e_int = q[3, 0, :, 5] - 0.5 * q[0, 0, :, 5] * vy * vy
+    np.testing.assert_allclose(e_int * 0.4, p_expect, rtol=1e-12)

@@ -9,11 +9,12 @@ from mosscap_amd import problems  # noqa: E402
 from tests import torch_hydro as th  # noqa: E402
 
 
-@pytest.mark.parametrize("solver", ["rusanov", "hllc"])
-def test_torch_step_matches_oracle(orc, solver):
+@pytest.mark.parametrize("solver,grav", [("rusanov", None), ("hllc", None), ("hllc", (-0.7, 0.3, 0.0))])
+def test_torch_step_matches_oracle(orc, solver, grav):
     cfg = problems.kh_config(24)
     cfg.ny = 20
     cfg.riemann_solver = solver
+    cfg.gravity = grav
     q0 = problems.kh_band(cfg, seed=3)
     sim = orc.OracleSim(cfg)
     sim.set_q(q0)
@@ -22,5 +23,5 @@ def test_torch_step_matches_oracle(orc, solver):
     dts = [2e-3, 1.5e-3, 2.5e-3]
     for dt in dts:
         sim.time_step(dt)
-        q = th.ssprk3_step(q, 1.4, dt, cfg.dx, cfg.ng, solver)
+        q = th.ssprk3_step(q, 1.4, dt, cfg.dx, cfg.ng, solver, grav=grav)
     np.testing.assert_allclose(q.numpy(), sim.get_q()[:, 0], rtol=1e-12, atol=1e-13)

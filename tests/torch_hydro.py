@@ -146,7 +146,15 @@ def stage_L(q, g, dtdx, ng, solver):
     return dtdx * (sweep(0) + sweep(1))
 
 
-def ssprk3_step(q, g, dt, dx, ng, solver):
+def gravity_source(q, grav, dt, ng):
+    """S*dt of the gravity term (Gravity.cpp:22-38) on the interior of q."""
+    inner = q[:, ng:-ng, ng:-ng]
+    zero = torch.zeros_like(inner[0])
+    return dt * torch.stack([zero, inner[0] * grav[0], inner[0] * grav[1],
+                             inner[1] * grav[0] + inner[2] * grav[1]])
+
+
+def ssprk3_step(q, g, dt, dx, ng, solver, grav=None):
     dtdx = dt / dx
 
     def with_interior(qbase, new_inner):
@@ -155,7 +163,12 @@ def ssprk3_step(q, g, dt, dx, ng, solver):
         return fill_periodic(out, ng)
 
     inner = lambda a: a[:, ng:-ng, ng:-ng]  # noqa: E731
-    q1 = with_interior(q, inner(q) + stage_L(q, g, dtdx, ng, solver))
-    q2 = with_interior(q, 0.75 * inner(q) + 0.25 * (inner(q1) + stage_L(q1, g, dtdx, ng, solver)))
-    q3 = with_interior(q, (1.0 / 3.0) * inner(q) + (2.0 / 3.0) * (inner(q2) + stage_L(q2, g, dtdx, ng, solver)))
+
+    def rhs(a):
+        r = stage_L(a, g, dtdx, ng, solver)
+        return r if grav is None else r + gravity_source(a, grav, dt, ng)
+
+    q1 = with_interior(q, inner(q) + rhs(q))
+    q2 = with_interior(q, 0.75 * inner(q) + 0.25 * (inner(q1) + rhs(q1)))
+    q3 = with_interior(q, (1.0 / 3.0) * inner(q) + (2.0 / 3.0) * (inner(q2) + rhs(q2)))
     return q3
