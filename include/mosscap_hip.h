@@ -174,6 +174,17 @@ void* mh_stream(mh_ctx* ctx);
  * enable = 0 removes it. The adjoint differentiates it. */
 int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz);
 
+/* ---- analytic LTE hydrogen EOS fields: eos.type = analyticlteh, the update_eos
+ * pass of AnalyticLteH.hpp:51-125 (Eos.cpp:23-27, 42-63) ---------------------
+ * Writes, for every cell (ghosts included, xc*yc values each), the ionisation
+ * fraction y and temperature T (bisection on T, Saha equation for y) of the
+ * state the reference's update_eos last saw: which = 1 takes the input of the
+ * most recent stage (update_eos runs at the start of each stage's fluxes,
+ * Hydro.cpp:325-328), which = 0 the current state. The hydro step does not use
+ * them. An inverted temperature bracket (the reference aborts) returns
+ * MH_ERR_INVALID. */
+int mh_eos_fields(mh_ctx* ctx, int32_t which, double avg_mass, int32_t ionisation_e, double* y, double* T);
+
 typedef int (*mh_user_bc_fn)(mh_ctx* ctx, void* user);
 int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user);
 

@@ -112,6 +112,7 @@ _SIGNATURES = [
     ("mh_stream", C.c_void_p, [C.c_void_p]),
     ("mh_set_user_bc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_set_gravity", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]),
+    ("mh_eos_fields", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p]),
     ("mh_device_state", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -238,6 +239,15 @@ class HydroContext:
         self.n_values = L.mh_num_values(h)
         if cfg.gravity is not None:
             self.set_gravity(cfg.gravity)
+
+    def eos_fields(self, which: int = 1, avg_mass: float = 1.0, ionisation_e: bool = False):
+        """Analytic LTE hydrogen (y, T) of every cell, shaped [z][y][x]: which=1 the most recent stage's
+        input (what update_eos saw), which=0 the current state."""
+        n = self.cfg.xc * self.cfg.yc
+        y, T = np.empty(n), np.empty(n)
+        self._c(lib().mh_eos_fields(self._h, which, avg_mass, int(ionisation_e), _ptr(y), _ptr(T)))
+        shape = (1, self.cfg.yc, self.cfg.xc)
+        return y.reshape(shape), T.reshape(shape)
 
     def set_gravity(self, g) -> None:
         """Register (gx, gy, gz) as a gravity source term, or remove it with None."""

@@ -187,6 +187,34 @@ int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz)
     return MH_OK;
 }
 
+int mh_eos_fields(mh_ctx* ctx, int32_t which, double avg_mass, int32_t ionisation_e, double* y, double* T) {
+    if (!ctx || !y || !T || (which != 0 && which != 1) || !(avg_mass > 0.0)) return MH_ERR_INVALID;
+    if (ctx->nd > 2) return ctx->fail(MH_ERR_UNSUPPORTED, "3-D grids are not built in this library");
+    hipError_t e = hipSetDevice(ctx->device);
+    const size_t n = (size_t)ctx->ncells;
+    if (e == hipSuccess && !ctx->eos_y) e = hipMalloc(&ctx->eos_y, n * sizeof(double));
+    if (e == hipSuccess && !ctx->eos_T) e = hipMalloc(&ctx->eos_T, n * sizeof(double));
+    if (e == hipSuccess && !ctx->eos_flag) e = hipMalloc(&ctx->eos_flag, sizeof(int));
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_eos_fields");
+    const int k = (which == 1 && ctx->last_in >= 0) ? ctx->last_in : ctx->cur;
+    const LteParams prm{ctx->gamma, 1.0 / avg_mass, ionisation_e ? 1 : 0};
+    int flag = 0;
+    e = hipMemsetAsync(ctx->eos_flag, 0, sizeof(int), ctx->stream);
+    if (e == hipSuccess) {
+        e = ctx->fp32c() ? launch_lte_eos_f32c(ctx->nd, ctx->p32(k), (int)ctx->ncells, prm, ctx->eos_y, ctx->eos_T,
+                                               ctx->eos_flag, ctx->stream)
+                         : launch_lte_eos_f64(ctx->nd, ctx->p64(k), (int)ctx->ncells, prm, ctx->eos_y, ctx->eos_T,
+                                              ctx->eos_flag, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, ctx->eos_y, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(T, ctx->eos_T, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->eos_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_eos_fields");
+    if (flag) return ctx->fail(MH_ERR_INVALID, "Temperature bounds flipped!");
+    return MH_OK;
+}
+
 int mh_device_state(mh_ctx* ctx, mh_device_view* out) {
     if (!ctx || !out) return MH_ERR_INVALID;
     const int k = ctx->view_buf >= 0 ? ctx->view_buf : ctx->cur;
@@ -215,6 +243,7 @@ static int upload_impl(mh_ctx* c, const double* q, hipMemcpyKind kind) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "mh_upload");
     c->next_stage = 0;
+    c->last_in = -1;
     return MH_OK;
 }
 
@@ -286,6 +315,7 @@ int mh_stage(mh_ctx* ctx, int32_t stage, double dt) {
     const int out = (in == ctx->qn) ? (ctx->qn + 1) % 3 : 3 - in - ctx->qn;
     if (e == hipSuccess) e = launch_clock_set_dt(ctx->clock, dt, ctx->stream);
     if (e == hipSuccess) e = ctx->stage(in, ctx->qn, out, stage_kind(ctx->sch.stepper, stage), 0);
+    ctx->last_in = in;
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_stage");
     ctx->cur = out;
@@ -335,7 +365,14 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
                 if (e != hipSuccess) break;
             }
             e = hipGraphLaunch(ctx->graph[k], ctx->stream);
-            ctx->cur = (k + (ctx->sch.stepper == TimeStepScheme::SspRk3 ? 1 : 2)) % 3;
+            {  // the buffers the captured step uses, as enqueue_step assigns them
+                int in = k;
+                for (int st = 0; st < num_stages(ctx->sch.stepper); ++st) {
+                    ctx->last_in = in;
+                    in = (in == k) ? (k + 1) % 3 : 3 - in - k;
+                }
+                ctx->cur = in;
+            }
         } else {
             e = ctx->finalize(1);
             if (e == hipSuccess) ctx->cur = ctx->enqueue_step(k, true, e);

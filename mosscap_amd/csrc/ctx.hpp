@@ -28,6 +28,10 @@ using namespace mh;
 struct mh_ctx {
     int nd = 0, nvar = 0;
     int ntr = 0;                     // tracer planes after the hydro variables
+    int last_in = -1;                // input buffer of the most recent stage (update_eos sees it)
+    double* eos_y = nullptr;         // analytic LTE EOS fields (mh_eos_fields)
+    double* eos_T = nullptr;
+    int* eos_flag = nullptr;
     int32_t gravity = 0;             // registered gravity source (mh_set_gravity)
     double grav[3] = {0.0, 0.0, 0.0};
     double* frx64 = nullptr;         // tracers: density fluxes of the stage (FP64 form)
@@ -228,6 +232,7 @@ struct mh_ctx {
         const bool fused = reduce_last && !user_fn;
         int in = k;
         for (int s = 0; s < ns; ++s) {
+            last_in = in;
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
             e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0);
             if (e != hipSuccess) return k;
@@ -258,10 +263,13 @@ struct mh_ctx {
         }
         for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
         ev_pool.clear();
-        for (void* t : {(void*)frx64, (void*)fry64, (void*)wtr64, (void*)frx32, (void*)fry32, (void*)wtr32})
+        for (void* t : {(void*)frx64, (void*)fry64, (void*)wtr64, (void*)frx32, (void*)fry32, (void*)wtr32,
+                        (void*)eos_y, (void*)eos_T, (void*)eos_flag})
             if (t) hipFree(t);
         frx64 = fry64 = wtr64 = nullptr;
         frx32 = fry32 = wtr32 = nullptr;
+        eos_y = eos_T = nullptr;
+        eos_flag = nullptr;
         if (staging) hipFree(staging);
         if (clock) hipFree(clock);
         if (floor_hits) hipFree(floor_hits);

@@ -195,17 +195,35 @@ Layout make_layout(const Simulation& sim, bool single_file) {
     L.dims.push_back({"time", single_file ? 0u : 1u});
     L.dims.push_back({"var", (uint32_t)sim.state.nvar});
     const uint64_t field = (uint64_t)sim.state.nvar * sz.zc * sz.yc * sz.xc;
+    const uint64_t cells = (uint64_t)sz.zc * sz.yc * sz.xc;
+    const bool eos_fields = !sim.eos.is_constant;  // ion_frac and T (Output.cpp:237-242, 261-266)
     if (single_file) {
         L.vars.push_back(Var{"time", {3}, 1, true});
         if (cfg.variables.conserved) L.vars.push_back(Var{"Q", {3, 4, 2, 1, 0}, field, true});
         if (cfg.variables.primitive) L.vars.push_back(Var{"W", {3, 4, 2, 1, 0}, field, true});
+        if (eos_fields) {
+            L.vars.push_back(Var{"ion_frac", {3, 2, 1, 0}, cells, true});
+            L.vars.push_back(Var{"T", {3, 2, 1, 0}, cells, true});
+        }
     } else {
         L.vars.push_back(Var{"time", {3}, 1, false});
         if (cfg.variables.conserved) L.vars.push_back(Var{"Q", {4, 2, 1, 0}, field, false});
         if (cfg.variables.primitive) L.vars.push_back(Var{"W", {4, 2, 1, 0}, field, false});
+        if (eos_fields) {
+            L.vars.push_back(Var{"ion_frac", {2, 1, 0}, cells, false});
+            L.vars.push_back(Var{"T", {2, 1, 0}, cells, false});
+        }
     }
     L.finalize();
     return L;
+}
+
+// EOS fields of the last update_eos (zeros before any, e.g. without a device)
+std::vector<double> eos_y(const Simulation& sim) {
+    return sim.eos.y_space.empty() ? std::vector<double>((size_t)sim.state.cells(), 0.0) : sim.eos.y_space;
+}
+std::vector<double> eos_T(const Simulation& sim) {
+    return sim.eos.T_space.empty() ? std::vector<double>((size_t)sim.state.cells(), 0.0) : sim.eos.T_space;
 }
 
 void write_at(std::fstream& f, uint64_t off, const std::vector<unsigned char>& bytes) {
@@ -253,6 +271,10 @@ bool write_output(Simulation& sim) {
                 vals = sim.state.Q;
             else if (v.name == "W")
                 vals = W;
+            else if (v.name == "ion_frac")
+                vals = eos_y(sim);
+            else if (v.name == "T")
+                vals = eos_T(sim);
             put_be_doubles(bytes, vals.data(), vals.size());
             write_at(f, v.begin, bytes);
         }
@@ -271,7 +293,10 @@ bool write_output(Simulation& sim) {
             if (v.name == "time") {
                 put_be_doubles(bytes, &sim.time, 1);
             } else {
-                const std::vector<double>& src = v.name == "Q" ? sim.state.Q : W;
+                const std::vector<double> src = v.name == "Q"          ? sim.state.Q
+                                                : v.name == "W"        ? W
+                                                : v.name == "ion_frac" ? eos_y(sim)
+                                                                       : eos_T(sim);
                 put_be_doubles(bytes, src.data(), src.size());
             }
             write_at(f, off, bytes);

@@ -143,16 +143,36 @@ void setup_timestepper(Simulation& sim, const Config& config) {
     sim.max_steps = get_or<long long>(config, "timestep.max_steps", -1);
 }
 
-// Eos::init, Eos.cpp:12-40 (ideal gas only).
+// Eos::init, Eos.cpp:12-63: the ideal gas and the analytic LTE hydrogen EOS.
 void setup_eos(Simulation& sim, const Config& config) {
     static const char* const EosTypeName[] = {"ideal", "analyticlteh", "tabulatedlteh", "dexpressure"};
     const std::string type = get_or<std::string>(config, "eos.type", "ideal");
     const int t = find_associated_enum<int>(EosTypeName, 4, type, "Mosscap::EosType");
-    if (t != 0) throw std::runtime_error("eos.type " + type + ": only the ideal-gas EOS is part of this build");
-    (void)get_or<double>(config, "eos.avg_mass", 1.0);
-    sim.eos.gamma = get_or<double>(config, "eos.gamma", 1.4);
-    (void)get_or<double>(config, "eos.ion_frac", 1.0);
-    sim.eos.is_constant = true;
+    sim.eos.avg_mass = get_or<double>(config, "eos.avg_mass", 1.0);
+    if (t == 0) {
+        sim.eos.gamma = get_or<double>(config, "eos.gamma", 1.4);
+        (void)get_or<double>(config, "eos.ion_frac", 1.0);
+        sim.eos.is_constant = true;
+    } else if (t == 1) {
+        sim.eos.gamma = get_or<double>(config, "eos.gamma", 5.0 / 3.0);
+        sim.eos.include_ionisation_energy = get_or<bool>(config, "eos.include_ionisation_energy", false);
+        sim.eos.is_constant = false;
+    } else {
+        throw std::runtime_error("eos.type " + type +
+                                 ": the tabulated and DexRT equations of state are not part of this build");
+    }
+}
+
+// update_eos (Hydro.cpp:325-328) for the analytic LTE EOS: fields of the state
+// the last stage started from (which = 1) or of the current state (which = 0).
+static void update_eos_fields(Simulation& sim, int which) {
+    if (sim.eos.is_constant) return;
+    const size_t n = (size_t)sim.state.cells();
+    sim.eos.y_space.resize(n);
+    sim.eos.T_space.resize(n);
+    check(sim, mh_eos_fields(sim.handle(), which, sim.eos.avg_mass, sim.eos.include_ionisation_energy ? 1 : 0,
+                             sim.eos.y_space.data(), sim.eos.T_space.data()),
+          "update_eos");
 }
 
 // setup_output, SetupModules.hpp:179-196.
@@ -306,6 +326,8 @@ void attach_device(Simulation& sim, bool write_initial_output) {
     }
     sim.ctx = std::shared_ptr<mh_ctx>(raw, [](mh_ctx* c) { mh_destroy(c); });
     check(sim, mh_upload(raw, sim.state.Q.data()), "upload");
+    // setup_sim runs update_eos on the problem's state before filling ghosts (SetupModules.hpp:250-255)
+    update_eos_fields(sim, 0);
 
     // select_hydro_fns / select_timestepper: the operator slots call the library.
     sim.compute_dt = [](const Simulation& s) {
@@ -317,6 +339,7 @@ void attach_device(Simulation& sim, bool write_initial_output) {
     };
     sim.time_step = [](Simulation& s, double dt) {
         check(s, mh_time_step(s.handle(), dt), "time_step");
+        update_eos_fields(s, 1);
         s.dt = dt;
         s.time += dt;  // TimeStepping.cpp:188-189
         s.current_step += 1;

@@ -72,9 +72,14 @@ struct NumericalSchemes {
     TimeStepScheme time_stepper = TimeStepScheme::SspRk3;
 };
 
+// Eos (Eos.hpp:27-47): ideal gas, or the analytic LTE hydrogen EOS whose
+// ionisation fraction and temperature fields are written with the output.
 struct Eos {
     double gamma = 1.4;
     bool is_constant = true;
+    double avg_mass = 1.0;
+    bool include_ionisation_energy = false;
+    std::vector<double> y_space, T_space;  // analytic LTE fields of the last update_eos
 };
 
 // OutputConfig (Output.hpp), read by setup_output (SetupModules.hpp:179-196).

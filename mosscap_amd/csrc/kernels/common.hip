@@ -188,6 +188,31 @@ hipError_t launch_ghost_f32c(int nd, const GhostGrid& g, FormF32c::Planes in, Fo
                              unsigned long long* amax, hipStream_t st) {
     return launch_ghost_impl<FormF32c>(nd, g, in, out, amax, st);
 }
+template <class Form>
+hipError_t launch_lte_impl(int nd, typename Form::Planes q, int ncells, const LteParams& p, double* y, double* T,
+                           int* flipped, hipStream_t st) {
+    if (ncells < 1 || !y || !T || !flipped) return hipErrorInvalidValue;
+    if (nd == 1) {
+        hipLaunchKernelGGL((lte_eos_kernel<Form, 1>), dim3(blocks_for(ncells)), dim3(kBlock), 0, st, q, ncells, p, y, T,
+                           flipped);
+    } else if (nd == 2) {
+        hipLaunchKernelGGL((lte_eos_kernel<Form, 2>), dim3(blocks_for(ncells)), dim3(kBlock), 0, st, q, ncells, p, y, T,
+                           flipped);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lte_eos_f64(int nd, FormF64::Planes q, int ncells, const LteParams& p, double* y, double* T,
+                              int* flipped, hipStream_t st) {
+    return launch_lte_impl<FormF64>(nd, q, ncells, p, y, T, flipped, st);
+}
+hipError_t launch_lte_eos_f32c(int nd, FormF32c::Planes q, int ncells, const LteParams& p, double* y, double* T,
+                               int* flipped, hipStream_t st) {
+    return launch_lte_impl<FormF32c>(nd, q, ncells, p, y, T, flipped, st);
+}
+
 hipError_t launch_tracer_ghost_f64(const GhostGrid& g, int ntr, FormF64::Planes in, FormF64::Planes out,
                                    hipStream_t st) {
     if (ntr < 1 || g.ng < 1 || g.xc < 3 * g.ng || g.yc < 3 * g.ng) return hipErrorInvalidValue;

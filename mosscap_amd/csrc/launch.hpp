@@ -11,6 +11,7 @@
 #include "device/ghost_dt.hpp"
 #include "device/stage_core.hpp"
 #include "device/tracer.hpp"
+#include "device/lte_eos.hpp"
 
 namespace mh {
 
@@ -48,6 +49,12 @@ hipError_t launch_cfl_f64(int nd, FormF64::Planes q, int ncells, double gamma, u
                           hipStream_t st);
 hipError_t launch_cfl_f32c(int nd, FormF32c::Planes q, int ncells, double gamma, unsigned long long* amax,
                            hipStream_t st);
+
+// Analytic LTE EOS fields (ionisation fraction, temperature) of every cell.
+hipError_t launch_lte_eos_f64(int nd, FormF64::Planes q, int ncells, const LteParams& p, double* y, double* T,
+                              int* flipped, hipStream_t st);
+hipError_t launch_lte_eos_f32c(int nd, FormF32c::Planes q, int ncells, const LteParams& p, double* y, double* T,
+                               int* flipped, hipStream_t st);
 
 hipError_t launch_clock_finalize(Clock* c, int advance, hipStream_t st);
 hipError_t launch_clock_set_dt(Clock* c, double dt, hipStream_t st);

@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
             "orc_slope_limiter": (d, [C.c_int, d, d]),
             "orc_reconstruct": (None, [C.c_int, C.c_int, i64, vp, vp, vp]),
             "orc_riemann": (None, [C.c_int, C.c_int, C.c_int, d, i64, vp, vp, vp]),
+            "orc_lte_eos": (C.c_int, [C.c_int, i64, vp, d, d, C.c_int, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -212,3 +213,16 @@ def riemann(solver, axis, nd, gamma, wl, wr):
     f = np.empty_like(a)
     lib().orc_riemann(SOLVER[solver], axis, nd, gamma, a.shape[0], _p(a), _p(b), _p(f))
     return f
+
+
+def lte_eos(nd, q, gamma, avg_mass=1.0, ionisation_e=False):
+    """Analytic LTE hydrogen EOS fields (ionisation fraction y, temperature T) of a state
+    q[var][...] (every cell), AnalyticLteH.hpp:51-125. Raises on an inverted bracket."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q[0].size
+    y, T = np.empty(n), np.empty(n)
+    rc = lib().orc_lte_eos(nd, n, _p(q), gamma, avg_mass, int(ionisation_e), _p(y), _p(T))
+    if rc:
+        raise RuntimeError("Temperature bounds flipped!")
+    return y.reshape(q[0].shape), T.reshape(q[0].shape)
+

@@ -674,6 +674,66 @@ int64_t orc_run(orc_sim* s, double max_cfl, double* time, double max_time, doubl
     return steps;
 }
 
+// ---- analytic LTE hydrogen EOS (AnalyticLteH.hpp:9-125) ---------------------
+// The update_eos pass of eos.type = analyticlteh: per cell, the temperature T
+// and ionisation fraction y of a pure-hydrogen gas in LTE for the cell's
+// density and internal energy, by bisection on T (Saha equation for y). The
+// hydro step does not read them; they are output fields. Q is [var][cells]
+// with nd + 2 hydro variables. Returns 1 if a cell's temperature bracket is
+// inverted (the reference aborts there), else 0.
+namespace {
+constexpr double kHMass = 1.6737830080950003e-27;  // [kg]
+constexpr double kKB = 1.380649e-23;               // [J/K]
+constexpr double kChiH = 2.178710282685096e-18;    // [J]
+
+double saha_rhs_h(double temp) {  // AnalyticLteH.hpp:9-13
+    return 2.4146830395719654e+21 * std::pow(temp, 1.5) * std::exp(-157763.42386247337 / temp);
+}
+double y_from_nhtot(double nhtot, double temp) {  // 15-19
+    const double X = saha_rhs_h(temp);
+    return 0.5 * (-X + std::sqrt(sq(X) + 4 * nhtot * X)) / nhtot;
+}
+}  // namespace
+
+int orc_lte_eos(int nd, int64_t ncells, const double* Q, double gamma, double avg_mass, int ionisation_e, double* y_out,
+                double* T_out) {
+    const double inv_avg_mass = 1.0 / avg_mass;
+    int flipped = 0;
+#pragma omp parallel for schedule(static) reduction(| : flipped)
+    for (int64_t c = 0; c < ncells; ++c) {
+        const double rho = Q[c];
+        double mom2_sum = sq(Q[ncells + c]);
+        if (nd > 1) mom2_sum += sq(Q[2 * ncells + c]);
+        if (nd > 2) mom2_sum += sq(Q[3 * ncells + c]);
+        const double e_kin = 0.5 * mom2_sum / rho;
+        const double eint = Q[(1 + nd) * ncells + c] - e_kin;
+        const double e_to_T = (gamma - 1.0) / (rho * (kKB / kHMass) * inv_avg_mass);
+        auto temp_from_y = [&](double y) {
+            return e_to_T / (1.0 + y) * (eint - (double)ionisation_e * y * rho * (kChiH / kHMass) * inv_avg_mass);
+        };
+        const double rho_to_nhtot = (1.0 / kHMass) * inv_avg_mass;
+        const double min_temperature = 100.0;
+        const double tb0 = std::max(temp_from_y(1.0), min_temperature), tb1 = temp_from_y(0.0);
+        if (tb0 > tb1) {
+            flipped = 1;
+            y_out[c] = T_out[c] = std::nan("");
+            continue;
+        }
+        double temp_step = tb1 - tb0, temp = tb0, y = 0.0;
+        for (int iter = 0; iter < 100; ++iter) {
+            temp_step *= 0.5;
+            const double test_temp = temp + temp_step;
+            y = y_from_nhtot(rho * rho_to_nhtot, test_temp);
+            const double temp_err = test_temp - temp_from_y(y);
+            if (temp_err <= 0.0) temp = test_temp;
+            if (std::abs(temp_step) < 1e-1 || std::abs(temp_err) < 1e-1) break;
+        }
+        y_out[c] = y;
+        T_out[c] = temp;
+    }
+    return flipped;
+}
+
 // ---- single-functor entry points for the unit tests ------------------------
 void orc_cons_to_prim(int nd, double gamma, const double* q, double* w) { cons_to_prim(gamma, q, w, nd); }
 void orc_prim_to_cons(int nd, double gamma, const double* w, double* q) { prim_to_cons(gamma, w, q, nd); }

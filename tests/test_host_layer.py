@@ -170,7 +170,7 @@ def test_constant_bc_mapping_1d(H):
     (MIN + "scheme:\n  riemann_solver: roe\n", r'Unable to find associated key in ".*RiemannSolver" for "roe".'),
     (MIN + "boundary:\n  xs: outflow\n", r'for "outflow"'),
     (MIN + "boundary:\n  xs: constant\n  xs_const: xxx\n", r"Provide constant boundary values for xs_const"),
-    (MIN + "eos:\n  type: tabulatedlteh\n", r"only the ideal-gas EOS"),
+    (MIN + "eos:\n  type: tabulatedlteh\n", r"tabulated and DexRT equations of state are not part"),
     (MIN + "output:\n  variables:\n    fluxes: true\n", r"fluxes / source"),
     ("simulation:\n  num_dim: 2\n", r"No method for circular_explosion found."),
     (MIN + "grid:\n  dx: -1.0\n", r"grid.dx cannot be negative."),
@@ -303,3 +303,15 @@ def test_kelvin_helmholtz_initial_state(H):
     vy = q[2, 0, :, 5] / q[0, 0, :, 5]
     e_int = q[3, 0, :, 5] - 0.5 * q[0, 0, :, 5] * vy * vy
     np.testing.assert_allclose(e_int * 0.4, p_expect, rtol=1e-12)
+
+
+def test_analytic_lte_eos_setup_and_output(H, tmp_path):
+    """eos.type analyticlteh: gamma defaults to 5/3 (Eos.cpp:24) and the output carries ion_frac and T
+    (zeros here: no device has run update_eos)."""
+    netcdf = pytest.importorskip("scipy.io").netcdf_file
+    text = MIN + "eos:\n  type: AnalyticLteH\n  avg_mass: 1.2\noutput:\n  name: lte\n"
+    assert H.setup_summary(text)["gamma"] == 5.0 / 3.0
+    H.write_initial_output(text, tmp_path)
+    with netcdf(tmp_path / "lte.nc", "r", mmap=False) as f:
+        assert f.variables["ion_frac"].shape == (2, 1, 262, 262)
+        assert f.variables["T"].shape == (2, 1, 262, 262)
