@@ -16,15 +16,17 @@ pytestmark = pytest.mark.gpu
 H_MASS, K_B = 1.6737830080950003e-27, 1.380649e-23
 
 
-def _solar_like(cfg, seed=5):
-    """Densities 1e-10..1e-6 kg/m^3 and temperatures 4e3..3e4 K (partly ionised hydrogen)."""
-    rng = np.random.default_rng(seed)
-    shape = (cfg.yc, cfg.xc)
-    rho = 10 ** rng.uniform(-10, -6, shape)
-    T = 10 ** rng.uniform(3.6, 4.5, shape)
+def _solar_like(cfg):
+    """Smooth fields: density 1e-9.5..1e-6.5 kg/m^3, temperature 4e3..2.5e4 K (partly ionised
+    hydrogen), a gentle flow."""
+    x = (np.arange(cfg.xc) + 0.5) / cfg.xc
+    y = (np.arange(cfg.yc) + 0.5) / cfg.yc
+    X, Y = np.meshgrid(x, y)
+    rho = 10 ** (-8.0 + 1.5 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y))
+    T = 10 ** (4.0 + 0.4 * np.sin(2 * np.pi * (X + Y)))
     p = rho * (K_B / H_MASS) * 1.5 * T
-    vx = rng.uniform(-1e3, 1e3, shape)
-    return problems.prim_to_cons(cfg, rho, vx, np.zeros(shape), p)
+    vx = 300.0 * np.cos(2 * np.pi * Y)
+    return problems.prim_to_cons(cfg, rho, vx, np.zeros_like(X), p)
 
 
 def _cfg():
