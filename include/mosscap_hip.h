@@ -25,7 +25,7 @@ extern "C" {
 
 #define MH_OK 0
 #define MH_ERR_INVALID 1     /* bad argument; the reference throws std::runtime_error (SetupModules.hpp:146) */
-#define MH_ERR_UNSUPPORTED 2 /* valid for the reference but outside this library (e.g. 3-D, 1-D tracers) */
+#define MH_ERR_UNSUPPORTED 2 /* valid for the reference but outside this library (e.g. 3-D userfn faces, 1-D tracers) */
 #define MH_ERR_HIP 3         /* HIP runtime or launch failure */
 
 #define MH_FORM_FP64 0  /* drop-in: FP64 state and arithmetic, bitwise the reference CPU build */
@@ -79,10 +79,11 @@ typedef struct {
 
 /* Build a context (replaces setup_grid's allocations, SetupModules.hpp:9-64,
  * plus select_hydro_fns / select_timestepper, Hydro.cpp:244-323 and
- * TimeStepping.cpp:331-344). num_dim 1 or 2. num_tracers (simulation.n_extra_fields,
- * 0..16, 2-D only) passive fields follow the hydro variables in Q: conserved
- * tracer densities, advected with the upwinded density flux (Hydro.cpp:27-33,
- * 155-162); constant boundaries are refused with tracers. */
+ * TimeStepping.cpp:331-344). num_dim 1, 2 or 3; 3-D contexts take every boundary
+ * type but userfn (the z faces use zs/ze and their constants). num_tracers
+ * (simulation.n_extra_fields, 0..16, 2-D only) passive fields follow the hydro
+ * variables in Q: conserved tracer densities, advected with the upwinded density
+ * flux (Hydro.cpp:27-33, 155-162); constant boundaries are refused with tracers. */
 int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const mh_boundaries* bdry,
               int32_t num_dim, int32_t num_tracers, const mh_schemes* schemes, double gamma,
               const mh_options* opts, mh_ctx** out);
@@ -176,7 +177,7 @@ int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz)
 
 /* ---- analytic LTE hydrogen EOS fields: eos.type = analyticlteh, the update_eos
  * pass of AnalyticLteH.hpp:51-125 (Eos.cpp:23-27, 42-63) ---------------------
- * Writes, for every cell (ghosts included, xc*yc values each), the ionisation
+ * Writes, for every cell (ghosts included, xc*yc*zc values each), the ionisation
  * fraction y and temperature T (bisection on T, Saha equation for y) of the
  * state the reference's update_eos last saw: which = 1 takes the input of the
  * most recent stage (update_eos runs at the start of each stage's fluxes,

@@ -174,12 +174,14 @@ class Config:
     num_dim: int = 2
     nx: int = 256
     ny: int = 256
+    nz: int = 1  # interior planes of a 3-D grid (grid.z)
     ng: int = 3
     x_dim: float = 1.0
     x_start: float = 0.0
     y_start: float = 0.0
-    bc: tuple = ("wall", "wall", "wall", "wall")  # xs, xe, ys, ye
-    bc_const: tuple = ((0.0,) * 5,) * 4
+    z_start: float = 0.0
+    bc: tuple = ("wall", "wall", "wall", "wall")  # xs, xe, ys, ye[, zs, ze]
+    bc_const: tuple = ((0.0,) * 5,) * 4  # conserved values per face, same order as bc
     reconstruction: str = "muscl"
     slope_limiter: str = "monotonizedcentral"
     riemann_solver: str = "hll"
@@ -200,6 +202,10 @@ class Config:
         return self.ny + 2 * self.ng if self.num_dim > 1 else 1
 
     @property
+    def zc(self) -> int:
+        return self.nz + 2 * self.ng if self.num_dim > 2 else 1
+
+    @property
     def nvar(self) -> int:
         return 2 + self.num_dim + self.num_tracers
 
@@ -209,7 +215,7 @@ class Config:
 
     @property
     def shape(self) -> tuple:
-        return (self.nvar, 1, self.yc, self.xc)
+        return (self.nvar, self.zc, self.yc, self.xc)
 
 
 class HydroContext:
@@ -219,14 +225,14 @@ class HydroContext:
                  rho_floor: float = 0.0, p_floor: float = 0.0):
         L = lib()
         self.cfg = cfg
-        sz = MhGridSize(cfg.xc, cfg.yc, 1, cfg.ng)
-        loc = MhGridLoc(cfg.x_start, cfg.y_start, 0.0)
+        sz = MhGridSize(cfg.xc, cfg.yc, cfg.zc, cfg.ng)
+        loc = MhGridLoc(cfg.x_start, cfg.y_start, cfg.z_start)
         b = MhBoundaries()
-        names = ("xs", "xe", "ys", "ye")
-        for n, kind, cv in zip(names, cfg.bc, cfg.bc_const):
+        names = ("xs", "xe", "ys", "ye", "zs", "ze")
+        for k, (n, kind) in enumerate(zip(names, cfg.bc)):
             setattr(b, n, BOUNDARY[kind.lower()])
             arr = getattr(b, n + "_const")
-            for i, v in enumerate(cv):
+            for i, v in enumerate(cfg.bc_const[k] if k < len(cfg.bc_const) else ()):
                 arr[i] = v
         sch = MhSchemes(RECONSTRUCTION[cfg.reconstruction], SLOPE_LIMITER[cfg.slope_limiter],
                         RIEMANN_SOLVER[cfg.riemann_solver], TIME_STEPPER[cfg.time_stepper])
@@ -243,10 +249,10 @@ class HydroContext:
     def eos_fields(self, which: int = 1, avg_mass: float = 1.0, ionisation_e: bool = False):
         """Analytic LTE hydrogen (y, T) of every cell, shaped [z][y][x]: which=1 the most recent stage's
         input (what update_eos saw), which=0 the current state."""
-        n = self.cfg.xc * self.cfg.yc
+        shape = (self.cfg.zc, self.cfg.yc, self.cfg.xc)
+        n = int(np.prod(shape))
         y, T = np.empty(n), np.empty(n)
         self._c(lib().mh_eos_fields(self._h, which, avg_mass, int(ionisation_e), _ptr(y), _ptr(T)))
-        shape = (1, self.cfg.yc, self.cfg.xc)
         return y.reshape(shape), T.reshape(shape)
 
     def set_gravity(self, g) -> None:

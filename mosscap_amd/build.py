@@ -50,6 +50,8 @@ COMMON = [
 UNITS = [
     ("kernels/stage_f64.hip", "off"),
     ("kernels/stage_f32c.hip", "fast"),
+    ("kernels/stage3d_f64.hip", "off"),
+    ("kernels/stage3d_f32c.hip", "fast"),
     ("kernels/common.hip", "off"),
     ("kernels/batch_f64.hip", "off"),
     ("kernels/batch_f32c.hip", "fast"),

@@ -33,7 +33,6 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
     if (num_dim < 1 || num_dim > 3) {
         return bad(MH_ERR_INVALID, "simulation.num_dim = " + std::to_string(num_dim) + ", must be in range [1, 3]");
     }
-    if (num_dim == 3) return bad(MH_ERR_UNSUPPORTED, "3-D grids are not built in this library");
     if (num_tracers < 0 || num_tracers > 16) return bad(MH_ERR_INVALID, "num_tracers must be in [0, 16]");
     if (num_tracers > 0 && num_dim != 2)
         return bad(MH_ERR_UNSUPPORTED, "tracer fields (simulation.n_extra_fields) are built for 2-D grids only");
@@ -52,18 +51,25 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
     }
     const int nx = sz->xc - 2 * sz->ng;
     const int ny = num_dim > 1 ? sz->yc - 2 * sz->ng : 1;
-    if (nx < sz->ng || ny < (num_dim > 1 ? sz->ng : 1) || sz->zc != 1 || (num_dim == 1 && sz->yc != 1)) {
+    const int nz = num_dim > 2 ? sz->zc - 2 * sz->ng : 1;
+    if (nx < sz->ng || ny < (num_dim > 1 ? sz->ng : 1) || nz < (num_dim > 2 ? sz->ng : 1) ||
+        (num_dim < 3 && sz->zc != 1) || (num_dim == 1 && sz->yc != 1)) {
         return bad(MH_ERR_INVALID, "grid size inconsistent with num_dim / num_ghost (need n >= num_ghost)");
     }
-    const int32_t bcs[4] = {bdry->xs, bdry->xe, bdry->ys, bdry->ye};
-    for (int k = 0; k < (num_dim > 1 ? 4 : 2); ++k) {
+    const int32_t bcs[6] = {bdry->xs, bdry->xe, bdry->ys, bdry->ye, bdry->zs, bdry->ze};
+    for (int k = 0; k < 2 * num_dim; ++k) {
         if (!valid_bc(bcs[k])) return bad(MH_ERR_INVALID, "unknown boundary type");
+        if (num_dim == 3 && bcs[k] == (int32_t)BoundaryType::UserFn)
+            return bad(MH_ERR_UNSUPPORTED, "userfn boundaries on 3-D grids");
     }
     if (bdry->xs == 1 ? bdry->xe != 1 : bdry->xe == 1) {
         return bad(MH_ERR_INVALID, "periodic boundaries must be paired (xs and xe)");
     }
     if (num_dim > 1 && (bdry->ys == 1 ? bdry->ye != 1 : bdry->ye == 1)) {
         return bad(MH_ERR_INVALID, "periodic boundaries must be paired (ys and ye)");
+    }
+    if (num_dim > 2 && (bdry->zs == 1 ? bdry->ze != 1 : bdry->ze == 1)) {
+        return bad(MH_ERR_INVALID, "periodic boundaries must be paired (zs and ze)");
     }
     if (!(dx > 0.0) || !(gamma > 1.0)) return bad(MH_ERR_INVALID, "dx must be > 0 and gamma > 1");
     if (num_tracers > 0) {
@@ -85,6 +91,7 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
     if (loc) c->loc = *loc;
     c->bdry = *bdry;
     if (num_dim == 1) c->bdry.ys = c->bdry.ye = (int32_t)BoundaryType::Periodic;  // unused
+    if (num_dim < 3) c->bdry.zs = c->bdry.ze = (int32_t)BoundaryType::Periodic;  // unused
     c->sch = Schemes{recon, (SlopeLimiter)schemes->slope_limit, (RiemannSolver)schemes->riemann_solver,
                      (TimeStepScheme)schemes->time_stepper};
     c->gamma = gamma;
@@ -102,7 +109,7 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
         // enough wavefronts to fill 256 CUs several times over, long spans
         // where the grid allows (the y halo costs 2R+2 rows per span)
         c->span_h = 32;
-        while (c->span_h > 8 && (int64_t)c->strips * ((ny + c->span_h - 1) / c->span_h) < 4096) c->span_h /= 2;
+        while (c->span_h > 8 && (int64_t)c->strips * ((ny + c->span_h - 1) / c->span_h) * nz < 4096) c->span_h /= 2;
     }
 
     hipError_t e = hipGetDevice(&c->device);
@@ -129,6 +136,11 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
             if (e == hipSuccess) e = hipMalloc(&c->fry64, n * sizeof(double));
             if (e == hipSuccess) e = hipMalloc(&c->wtr64, n * num_tracers * sizeof(double));
         }
+    }
+    if (e == hipSuccess && num_dim == 3) {
+        const size_t n = (size_t)ncells * 5;
+        if (c->fp32c()) e = hipMalloc(&c->dz32, n * sizeof(float));
+        else e = hipMalloc(&c->dz64, n * sizeof(double));
     }
     if (e == hipSuccess) e = hipMalloc(&c->clock, sizeof(Clock));
     if (e == hipSuccess) e = hipMalloc(&c->floor_hits, sizeof(unsigned int));
@@ -177,19 +189,18 @@ int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user) {
 int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz) {
     if (!ctx) return MH_ERR_INVALID;
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
-    if (enable && (gz != 0.0 || (ctx->nd < 2 && gy != 0.0)))
+    if (enable && ((ctx->nd < 3 && gz != 0.0) || (ctx->nd < 2 && gy != 0.0)))
         return ctx->fail(MH_ERR_INVALID, "gravity components beyond the grid's dimensions must be 0");
     ctx->gravity = enable ? 1 : 0;
     ctx->grav[0] = enable ? gx : 0.0;
     ctx->grav[1] = enable ? gy : 0.0;
-    ctx->grav[2] = 0.0;
+    ctx->grav[2] = enable ? gz : 0.0;
     ctx->drop_graphs();
     return MH_OK;
 }
 
 int mh_eos_fields(mh_ctx* ctx, int32_t which, double avg_mass, int32_t ionisation_e, double* y, double* T) {
     if (!ctx || !y || !T || (which != 0 && which != 1) || !(avg_mass > 0.0)) return MH_ERR_INVALID;
-    if (ctx->nd > 2) return ctx->fail(MH_ERR_UNSUPPORTED, "3-D grids are not built in this library");
     hipError_t e = hipSetDevice(ctx->device);
     const size_t n = (size_t)ctx->ncells;
     if (e == hipSuccess && !ctx->eos_y) e = hipMalloc(&ctx->eos_y, n * sizeof(double));
@@ -561,7 +572,7 @@ int mh_probe_reconstruct(int32_t recon, int32_t limiter, int32_t prec, int64_t n
 
 int mh_probe_riemann(int32_t solver, int32_t axis, int32_t nd, int32_t prec, double gamma, int64_t n,
                      const double* wl, const double* wr, double* flux) {
-    if (n < 1 || n > (1 << 26) || !wl || !wr || !flux || nd < 1 || nd > 2 || axis < 0 || axis >= nd) return MH_ERR_INVALID;
+    if (n < 1 || n > (1 << 26) || !wl || !wr || !flux || nd < 1 || nd > 3 || axis < 0 || axis >= nd) return MH_ERR_INVALID;
     const size_t m = (size_t)n * (2 + nd);
     DevBuf a, b, f;
     hipError_t e = a.alloc(m);

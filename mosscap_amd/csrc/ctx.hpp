@@ -40,6 +40,8 @@ struct mh_ctx {
     float* frx32 = nullptr;          // ... compensated form
     float* fry32 = nullptr;
     float* wtr32 = nullptr;
+    double* dz64 = nullptr;          // 3-D: z flux differences of the stage (FP64 form)
+    float* dz32 = nullptr;           // ... compensated form
     mh_grid_size sz{};
     double dx = 0.0;
     mh_grid_loc loc{};
@@ -141,9 +143,30 @@ struct mh_ctx {
         return g;
     }
 
+    Ghost3Grid ghost3_grid() const {
+        Ghost3Grid g{};
+        g.xc = sz.xc;
+        g.yc = sz.yc;
+        g.zc = sz.zc;
+        g.ng = sz.ng;
+        const int32_t b[6] = {bdry.xs, bdry.xe, bdry.ys, bdry.ye, bdry.zs, bdry.ze};
+        const double* cv[6] = {bdry.xs_const, bdry.xe_const, bdry.ys_const, bdry.ye_const, bdry.zs_const,
+                               bdry.ze_const};
+        for (int f = 0; f < 6; ++f) {
+            g.bc[f] = b[f];
+            std::memcpy(g.bconst[f], cv[f], sizeof(double) * 5);
+        }
+        return g;
+    }
+
+    // Ghost fill of `out` after a stage from `in`. In 3-D the three axis
+    // passes run on `out` alone and the CFL fold is a separate pass.
     hipError_t ghost(int in, int out, int reduce) {
         unsigned long long* amax = &clock->amax_bits;
         hipError_t e = timed(1, [&] {
+            if (nd == 3)
+                return fp32c() ? launch_ghost3d_f32c(ghost3_grid(), p32(out), stream)
+                               : launch_ghost3d_f64(ghost3_grid(), p64(out), stream);
             hipError_t r = fp32c() ? launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream)
                                    : launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
             if (r != hipSuccess || ntr == 0) return r;
@@ -174,6 +197,15 @@ struct mh_ctx {
         return ghost(in, out, reduce);
     }
 
+    // 3-D march grid: no CFL fold inside (the 3-D ghost passes do not fold, so
+    // the whole grid is reduced after the last stage instead)
+    StageGrid grid3(int kind) const {
+        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, 0};
+        g.zc = sz.zc;
+        g.nz = sz.zc - 2 * sz.ng;
+        return g;
+    }
+
     hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce) {
         hipError_t e;
         const TracerGrid tg{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, ntr, kind};
@@ -185,6 +217,9 @@ struct mh_ctx {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
                 e = launch_stage2d_f32c(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
+            } else if (nd == 3) {
+                p.gz = grav[2];
+                e = launch_stage3d_f32c(sch, grid3(kind), p, dz32, stream);
             } else {
                 e = launch_stage1d_f32c(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
@@ -195,6 +230,9 @@ struct mh_ctx {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
                 e = launch_stage2d_f64(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
+            } else if (nd == 3) {
+                p.gz = grav[2];
+                e = launch_stage3d_f64(sch, grid3(kind), p, dz64, stream);
             } else {
                 e = launch_stage1d_f64(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
@@ -226,10 +264,10 @@ struct mh_ctx {
     }
 
     // All stages of one step starting from `cur`; returns the new cur.
-    // With a user_bc hook the reduction is a separate pass after the hook.
+    // With a user_bc hook, and in 3-D, the reduction is a separate pass at the end.
     int enqueue_step(int k, bool reduce_last, hipError_t& e) {
         const int ns = num_stages(sch.stepper);
-        const bool fused = reduce_last && !user_fn;
+        const bool fused = reduce_last && !user_fn && nd != 3;
         int in = k;
         for (int s = 0; s < ns; ++s) {
             last_in = in;
@@ -238,7 +276,7 @@ struct mh_ctx {
             if (e != hipSuccess) return k;
             in = out;
         }
-        if (reduce_last && user_fn) e = cfl_all(in);
+        if (reduce_last && !fused) e = cfl_all(in);
         return in;
     }
 
@@ -264,8 +302,10 @@ struct mh_ctx {
         for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
         ev_pool.clear();
         for (void* t : {(void*)frx64, (void*)fry64, (void*)wtr64, (void*)frx32, (void*)fry32, (void*)wtr32,
-                        (void*)eos_y, (void*)eos_T, (void*)eos_flag})
+                        (void*)eos_y, (void*)eos_T, (void*)eos_flag, (void*)dz64, (void*)dz32})
             if (t) hipFree(t);
+        dz64 = nullptr;
+        dz32 = nullptr;
         frx64 = fry64 = wtr64 = nullptr;
         frx32 = fry32 = wtr32 = nullptr;
         eos_y = eos_T = nullptr;

@@ -88,6 +88,8 @@ struct FormF32c {
         float* fry = nullptr;  // ... of y faces [j][i] (face j of column i)
         int32_t gravity = 0;   // a gravity source term is registered
         double gx = 0.0, gy = 0.0;
+        const float* dz3 = nullptr;  // 3-D: z flux difference per interior cell [var][k][j][i]
+        double gz = 0.0;
     };
 
     __device__ static inline float2 get(const Planes& a, int idx) { return make_float2(a.hi[idx], a.lo[idx]); }
@@ -153,6 +155,10 @@ struct FormF32c {
         if constexpr (ND > 1) {
             src[2] = (float)(rho * p.gy * d);
             energy_update += df::to_double(q[2]) * p.gy;
+        }
+        if constexpr (ND > 2) {
+            src[3] = (float)(rho * p.gz * d);
+            energy_update += df::to_double(q[3]) * p.gz;
         }
         src[1 + ND] = (float)(energy_update * d);
     }

@@ -25,6 +25,8 @@ struct FormF64 {
         double* fry = nullptr;          // ... of y faces [j][i] (face j of column i)
         int32_t gravity = 0;            // a gravity source term is registered
         double gx = 0.0, gy = 0.0;      // its acceleration
+        const double* dz3 = nullptr;    // 3-D: z flux difference Fz_k - Fz_{k+1} per interior cell [var][k][j][i]
+        double gz = 0.0;
     };
 
     __device__ static inline double get(const Planes& a, int idx) { return a.p[idx]; }
@@ -77,6 +79,10 @@ struct FormF64 {
         if constexpr (ND > 1) {
             src[2] = (0.0 + q[0] * p.gy) * dt;
             energy_update += q[2] * p.gy;
+        }
+        if constexpr (ND > 2) {
+            src[3] = (0.0 + q[0] * p.gz) * dt;
+            energy_update += q[3] * p.gz;
         }
         src[1 + ND] = (0.0 + energy_update) * dt;
     }

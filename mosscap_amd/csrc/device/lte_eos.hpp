@@ -39,6 +39,7 @@ __global__ __launch_bounds__(256) void lte_eos_kernel(typename Form::Planes q, i
     const double rho = Form::value(Form::get(q, c));
     double mom2_sum = square(Form::value(Form::get(q, ncells + c)));
     if constexpr (ND > 1) mom2_sum += square(Form::value(Form::get(q, 2 * ncells + c)));
+    if constexpr (ND > 2) mom2_sum += square(Form::value(Form::get(q, 3 * ncells + c)));
     const double e_kin = 0.5 * mom2_sum / rho;
     const double eint = Form::value(Form::get(q, (1 + ND) * ncells + c)) - e_kin;
     const double e_to_T = (prm.gamma - 1.0) / (rho * (lte::kKB / lte::kHMass) * prm.inv_avg_mass);

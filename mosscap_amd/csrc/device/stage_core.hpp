@@ -51,13 +51,18 @@ struct StageGrid {
     int32_t strips, span_h;
     int32_t kind;        // StageKind
     int32_t reduce_dt;   // fold the new state into the CFL max
+    int32_t zc = 1, nz = 1;  // 3-D: planes (ghosts included) and interior planes
 };
 
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+// The x-y march of one plane. ND = 2: the whole 2-D stage. ND = 3: one
+// interior z-plane per (strip, span, plane) workgroup; the z flux difference
+// of each cell comes from the z sweep (p.dz3) and is added after the y one,
+// as integrate_flux adds Fz after Fy (TimeStepping.cpp:22-30).
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int ND = 2>
 __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Ptrs& p) {
     using T = typename Form::real;
     using Raw = typename Form::raw;
-    constexpr int NV = 4;  // 2-D ideal gas: rho, mx, my, E
+    constexpr int NV = 2 + ND;  // rho, m_x, m_y[, m_z], E
     constexpr int ORD = stencil_order(R);
     constexpr int WIN = 2 * ORD + 1;
     constexpr int ROW = kWave + 2 * ORD;
@@ -67,7 +72,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     __shared__ T ring[RING][NV][ROW];
 
     const int lane = threadIdx.x;
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t bb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t per_plane = (uint32_t)g.strips * (uint32_t)((g.ny + g.span_h - 1) / g.span_h);
+    const uint32_t b = bb % per_plane;
     const int strip = (int)(b % (uint32_t)g.strips);
     const int span = (int)(b / (uint32_t)g.strips);
     const int i0 = g.ng + strip * kStripOut;
@@ -76,7 +83,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int c = i0 - 1 + lane;
     const int c_ld = min(c, g.xc - 1);
     const int xc = g.xc;
-    const int plane = g.xc * g.yc;
+    // variables are [var][k][j][i]: var stride of the whole grid, then this plane's offset
+    const int plane = g.xc * g.yc * (ND > 2 ? g.zc : 1);
+    const int pk = ND > 2 ? (g.ng + (int)(bb / per_plane)) * g.xc * g.yc : 0;
 
     // Halo lanes: lanes [0, ORD) fill entries [0, ORD) (columns left of the
     // strip), lanes [ORD, 2*ORD) fill entries [64+ORD, 64+2*ORD).
@@ -96,7 +105,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     auto commit_row = [&](int r, const Raw (&q)[NV], const Raw (&qh)[NV]) {
         const int slot = r & (RING - 1);
         T w[NV];
-        const bool hit = Form::template to_prim_nd<2>(p, q, w);
+        const bool hit = Form::template to_prim_nd<ND>(p, q, w);
         // each interior cell is committed by exactly one (strip, span) owner
         floor_hits += (hit && out_lane && r >= j0 && r < j1) ? 1u : 0u;
 #pragma unroll
@@ -109,13 +118,13 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         for (int v = 0; v < NV; ++v) win[WIN - 1][v] = w[v];
         if (ORD > 0 && halo_lane) {
             T wh[NV];
-            Form::template to_prim_nd<2>(p, qh, wh);
+            Form::template to_prim_nd<ND>(p, qh, wh);
 #pragma unroll
             for (int v = 0; v < NV; ++v) ring[slot][v][h_entry] = wh[v];
         }
     };
     auto fetch_row = [&](int r, Raw (&q)[NV], Raw (&qh)[NV]) {
-        const int base = r * xc;
+        const int base = pk + r * xc;
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = Form::get(p.qs, v * plane + base + c_ld);
         if (ORD > 0 && halo_lane) {
@@ -129,8 +138,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             auto s = [&](int o) -> T { return win[ORD + o][v]; };
             reconstruct_stencil<R, SL, T>(s, wl[v], wr[v]);
         }
-        Form::template floor_face<2>(p, wl);
-        Form::template floor_face<2>(p, wr);
+        Form::template floor_face<ND>(p, wl);
+        Form::template floor_face<ND>(p, wr);
     };
 
     // ---- prologue: rows j0-1-ORD .. j0+ORD; y faces up to j0 ------------------
@@ -149,8 +158,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         fetch_row(j0 + ORD, qb, qhb);
         commit_row(j0 + ORD, qb, qhb);
         recon_y(yl, yr);  // cell j0
-        riemann_flux<RS, 1, 2>(eos, yRp, yl, Gyp);
-        if (p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
+        riemann_flux<RS, 1, ND>(eos, yRp, yl, Gyp);
+        if (ND == 2 && p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
         if (j0 < j1) fetch_row(j0 + 1 + ORD, qa, qha);
@@ -161,7 +170,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     for (int j = j0; j < j1; ++j) {
         // Stage-combine operands for this row, issued early.
         Raw qs_c[NV], qn_c[NV];
-        const int cell = j * xc + c_ld;
+        const int cell = pk + j * xc + c_ld;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             qs_c[v] = Form::get(p.qs, v * plane + cell);
@@ -174,8 +183,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         // y sweep: cell j+1, face j+1
         T yl[NV], yr[NV], Gy[NV], dy[NV];
         recon_y(yl, yr);
-        riemann_flux<RS, 1, 2>(eos, yRp, yl, Gy);
-        if (p.fry != nullptr && out_lane) p.fry[(j + 1) * xc + c] = Gy[0];
+        riemann_flux<RS, 1, ND>(eos, yRp, yl, Gy);
+        if (ND == 2 && p.fry != nullptr && out_lane) p.fry[(j + 1) * xc + c] = Gy[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             dy[v] = Gyp[v] - Gy[v];
@@ -192,12 +201,12 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             auto s = [&](int o) -> T { return ring[slot][v][lane + ORD + o]; };
             reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
         }
-        Form::template floor_face<2>(p, xl);
-        Form::template floor_face<2>(p, xr);
+        Form::template floor_face<ND>(p, xl);
+        Form::template floor_face<ND>(p, xr);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
-        riemann_flux<RS, 0, 2>(eos, xrm, xl, Fx);
-        if (p.frx != nullptr && lane >= 1 && c >= g.ng && c <= g.ng + g.nx) p.frx[j * xc + c] = Fx[0];
+        riemann_flux<RS, 0, ND>(eos, xrm, xl, Fx);
+        if (ND == 2 && p.frx != nullptr && lane >= 1 && c >= g.ng && c <= g.ng + g.nx) p.frx[j * xc + c] = Fx[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 
@@ -205,7 +214,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             Raw qo[NV];
             T srcv[NV];
             if (p.gravity) {
-                Form::template gravity_src<2>(p, qs_c, dt, srcv);
+                Form::template gravity_src<ND>(p, qs_c, dt, srcv);
             } else {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) srcv[v] = src;
@@ -215,12 +224,13 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 T q_update = T(0);
                 q_update += Fx[v] - Fxp[v];
                 q_update += dy[v];
+                if constexpr (ND > 2) q_update += p.dz3[v * plane + pk + j * xc + c];
                 q_update *= dtdx;
                 qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v]);
-                Form::put(p.out, v * plane + j * xc + c, qo[v]);
+                Form::put(p.out, v * plane + pk + j * xc + c, qo[v]);
             }
             if (g.reduce_dt) {
-                const double a = Form::template signal_speed_nd<2>((double)p.gamma, qo);
+                const double a = Form::template signal_speed_nd<ND>((double)p.gamma, qo);
                 amax = a > amax ? a : amax;
             }
         }

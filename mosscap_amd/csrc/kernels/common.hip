@@ -27,6 +27,17 @@ hipError_t launch_ghost_impl(int nd, const GhostGrid& g, typename Form::Planes i
     return hipGetLastError();
 }
 
+// 3-D ghost fill: the x, y and z passes in order on one buffer.
+template <class Form>
+hipError_t launch_ghost3d_impl(const Ghost3Grid& g, typename Form::Planes q, hipStream_t st) {
+    if (g.ng < 1 || g.xc < 3 * g.ng || g.yc < 3 * g.ng || g.zc < 3 * g.ng) return hipErrorInvalidValue;
+    const long long nx = 2LL * g.ng * g.yc * g.zc, ny = 2LL * g.ng * g.xc * g.zc, nz = 2LL * g.ng * g.xc * g.yc;
+    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 0>), dim3(blocks_for(nx)), dim3(kBlock), 0, st, g, q);
+    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 1>), dim3(blocks_for(ny)), dim3(kBlock), 0, st, g, q);
+    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 2>), dim3(blocks_for(nz)), dim3(kBlock), 0, st, g, q);
+    return hipGetLastError();
+}
+
 template <class Form>
 hipError_t launch_cfl_impl(int nd, typename Form::Planes q, int ncells, double gamma, unsigned long long* amax,
                            hipStream_t st) {
@@ -36,6 +47,8 @@ hipError_t launch_cfl_impl(int nd, typename Form::Planes q, int ncells, double g
         hipLaunchKernelGGL((cfl_reduce_kernel<Form, 1>), dim3(blocks), dim3(kBlock), 0, st, q, ncells, gamma, amax);
     } else if (nd == 2) {
         hipLaunchKernelGGL((cfl_reduce_kernel<Form, 2>), dim3(blocks), dim3(kBlock), 0, st, q, ncells, gamma, amax);
+    } else if (nd == 3) {
+        hipLaunchKernelGGL((cfl_reduce_kernel<Form, 3>), dim3(blocks), dim3(kBlock), 0, st, q, ncells, gamma, amax);
     } else {
         return hipErrorInvalidValue;
     }
@@ -129,6 +142,9 @@ hipError_t probe_riemann_rs(int axis, int nd, double gamma, int n, const double*
     if (nd == 1 && axis == 0) hipLaunchKernelGGL((probe_riemann_kernel<T, RS, 0, 1>), grid, block, 0, st, gamma, n, wl, wr, f);
     else if (nd == 2 && axis == 0) hipLaunchKernelGGL((probe_riemann_kernel<T, RS, 0, 2>), grid, block, 0, st, gamma, n, wl, wr, f);
     else if (nd == 2 && axis == 1) hipLaunchKernelGGL((probe_riemann_kernel<T, RS, 1, 2>), grid, block, 0, st, gamma, n, wl, wr, f);
+    else if (nd == 3 && axis == 0) hipLaunchKernelGGL((probe_riemann_kernel<T, RS, 0, 3>), grid, block, 0, st, gamma, n, wl, wr, f);
+    else if (nd == 3 && axis == 1) hipLaunchKernelGGL((probe_riemann_kernel<T, RS, 1, 3>), grid, block, 0, st, gamma, n, wl, wr, f);
+    else if (nd == 3 && axis == 2) hipLaunchKernelGGL((probe_riemann_kernel<T, RS, 2, 3>), grid, block, 0, st, gamma, n, wl, wr, f);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -188,6 +204,12 @@ hipError_t launch_ghost_f32c(int nd, const GhostGrid& g, FormF32c::Planes in, Fo
                              unsigned long long* amax, hipStream_t st) {
     return launch_ghost_impl<FormF32c>(nd, g, in, out, amax, st);
 }
+hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, hipStream_t st) {
+    return launch_ghost3d_impl<FormF64>(g, q, st);
+}
+hipError_t launch_ghost3d_f32c(const Ghost3Grid& g, FormF32c::Planes q, hipStream_t st) {
+    return launch_ghost3d_impl<FormF32c>(g, q, st);
+}
 template <class Form>
 hipError_t launch_lte_impl(int nd, typename Form::Planes q, int ncells, const LteParams& p, double* y, double* T,
                            int* flipped, hipStream_t st) {
@@ -197,6 +219,9 @@ hipError_t launch_lte_impl(int nd, typename Form::Planes q, int ncells, const Lt
                            flipped);
     } else if (nd == 2) {
         hipLaunchKernelGGL((lte_eos_kernel<Form, 2>), dim3(blocks_for(ncells)), dim3(kBlock), 0, st, q, ncells, p, y, T,
+                           flipped);
+    } else if (nd == 3) {
+        hipLaunchKernelGGL((lte_eos_kernel<Form, 3>), dim3(blocks_for(ncells)), dim3(kBlock), 0, st, q, ncells, p, y, T,
                            flipped);
     } else {
         return hipErrorInvalidValue;

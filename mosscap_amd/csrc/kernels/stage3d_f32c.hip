@@ -1,0 +1,15 @@
+// 3-D stage kernels (z sweep + per-plane x-y march) of the compensated FP32
+// form, built with -ffp-contract=fast like the 2-D unit. Five variables per
+// cell need more registers than the 2-D march: two waves per SIMD.
+#define MH_STAGE_WAVES_PER_EU 2
+#define MH_STAGE3D_INSTANCES 1
+#include "stage_inst.hpp"
+
+namespace mh {
+
+hipError_t launch_stage3d_f32c(const Schemes& s, const StageGrid& g, const FormF32c::Ptrs& p, float* dz,
+                               hipStream_t st) {
+    return launch_stage3d_impl<FormF32c>(s, g, p, dz, st);
+}
+
+}  // namespace mh

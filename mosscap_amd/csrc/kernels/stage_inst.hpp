@@ -113,6 +113,56 @@ hipError_t launch_stage1d_impl(const Schemes& s, int kind, int reduce_dt, int xc
     return hipGetLastError();
 }
 
+#ifdef MH_STAGE3D_INSTANCES
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MH_STAGE_WAVES_PER_EU)))
+void stage3d_kernel(StageGrid g, typename Form::Ptrs p) {
+    stage2d_march<Form, R, SL, RS, 3>(g, p);
+}
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__global__ __launch_bounds__(128) void zsweep_kernel(ZSweepGrid g, typename Form::Ptrs p, typename Form::real* dz) {
+    zsweep_column<Form, R, SL, RS>(g, p, dz);
+}
+
+template <class Form>
+hipError_t launch_stage3d_impl(const Schemes& s, const StageGrid& g, const typename Form::Ptrs& p,
+                               typename Form::real* dz, hipStream_t st) {
+    if (g.nx < 1 || g.ny < 1 || g.nz < 1 || g.ng < min_ghost_cells(s.recon) || g.xc != g.nx + 2 * g.ng ||
+        g.yc != g.ny + 2 * g.ng || g.zc != g.nz + 2 * g.ng || g.span_h < 1 ||
+        g.strips != (g.nx + kStripOut - 1) / kStripOut || !dz) {
+        return hipErrorInvalidValue;
+    }
+    using ZFn = void (*)(ZSweepGrid, typename Form::Ptrs, typename Form::real*);
+    auto zpick = [&]<Reconstruction R, SlopeLimiter SL>() -> ZFn {
+        switch (s.solver) {
+            case RiemannSolver::Rusanov: return zsweep_kernel<Form, R, SL, RiemannSolver::Rusanov>;
+            case RiemannSolver::Hll: return zsweep_kernel<Form, R, SL, RiemannSolver::Hll>;
+            default: return zsweep_kernel<Form, R, SL, RiemannSolver::Hllc>;
+        }
+    };
+    auto mpick = [&]<Reconstruction R, SlopeLimiter SL>() -> Stage2dFn<Form> {
+        switch (s.solver) {
+            case RiemannSolver::Rusanov: return stage3d_kernel<Form, R, SL, RiemannSolver::Rusanov>;
+            case RiemannSolver::Hll: return stage3d_kernel<Form, R, SL, RiemannSolver::Hll>;
+            default: return stage3d_kernel<Form, R, SL, RiemannSolver::Hllc>;
+        }
+    };
+    // z sweep: chunks of at most 32 planes per thread, columns of the interior across the grid's x
+    const int chunks = (g.nz + 31) / 32;
+    const ZSweepGrid zg{g.xc, g.yc, g.zc, g.ng, g.nx, g.ny, g.nz, (g.nz + chunks - 1) / chunks};
+    const int cols = g.nx * g.ny;
+    hipLaunchKernelGGL(pick_scheme<Form>(s, zpick), dim3((unsigned)((cols + 127) / 128), (unsigned)chunks), dim3(128),
+                       0, st, zg, p, dz);
+    typename Form::Ptrs pm = p;
+    pm.dz3 = dz;
+    const int spans = (g.ny + g.span_h - 1) / g.span_h;
+    hipLaunchKernelGGL(pick_scheme<Form>(s, mpick), dim3((unsigned)(g.strips * spans * g.nz)), dim3(kWave), 0, st, g,
+                       pm);
+    return hipGetLastError();
+}
+#endif
+
 // Tracer planes of one 2-D stage (after the hydro stage stored the density
 // fluxes): W_t of the stage input, then the interior update.
 template <class Form>

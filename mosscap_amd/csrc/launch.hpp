@@ -10,6 +10,7 @@
 #include "device/form_f32c.hpp"
 #include "device/ghost_dt.hpp"
 #include "device/stage_core.hpp"
+#include "device/stage3d.hpp"
 #include "device/tracer.hpp"
 #include "device/lte_eos.hpp"
 
@@ -25,6 +26,12 @@ struct Schemes {
 // 2-D fused stage over the interior, one wavefront per (strip, span).
 hipError_t launch_stage2d_f64(const Schemes& s, const StageGrid& g, const FormF64::Ptrs& p, hipStream_t st);
 hipError_t launch_stage2d_f32c(const Schemes& s, const StageGrid& g, const FormF32c::Ptrs& p, hipStream_t st);
+// 3-D stage: the z sweep into `dz` (NV values per cell, [var][k][j][i]),
+// then the x-y march of every interior plane (g.zc, g.nz set).
+hipError_t launch_stage3d_f64(const Schemes& s, const StageGrid& g, const FormF64::Ptrs& p, double* dz,
+                              hipStream_t st);
+hipError_t launch_stage3d_f32c(const Schemes& s, const StageGrid& g, const FormF32c::Ptrs& p, float* dz,
+                               hipStream_t st);
 // 1-D stage over one row of xc cells (ng ghosts each side).
 hipError_t launch_stage1d_f64(const Schemes& s, int kind, int reduce_dt, int xc, int ng, const FormF64::Ptrs& p,
                               hipStream_t st);
@@ -45,6 +52,9 @@ hipError_t launch_ghost_f64(int nd, const GhostGrid& g, FormF64::Planes in, Form
                             unsigned long long* amax, hipStream_t st);
 hipError_t launch_ghost_f32c(int nd, const GhostGrid& g, FormF32c::Planes in, FormF32c::Planes out,
                              unsigned long long* amax, hipStream_t st);
+// 3-D ghost fill of one buffer (x, y, z passes).
+hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, hipStream_t st);
+hipError_t launch_ghost3d_f32c(const Ghost3Grid& g, FormF32c::Planes q, hipStream_t st);
 hipError_t launch_cfl_f64(int nd, FormF64::Planes q, int ncells, double gamma, unsigned long long* amax,
                           hipStream_t st);
 hipError_t launch_cfl_f32c(int nd, FormF32c::Planes q, int ncells, double gamma, unsigned long long* amax,

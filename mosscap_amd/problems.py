@@ -116,6 +116,53 @@ def sedov(cfg: Config, energy: float = 1.0, radius_cells: float = 3.5, p_ambient
     return prim_to_cons(cfg, rho, zero, zero, p)
 
 
+def cell_centres_3d(cfg: Config):
+    """State::get_pos of every cell of a 3-D grid: x[i], y[j], z[k]."""
+    off = -(cfg.ng - 0.5) * cfg.dx
+    x = np.arange(cfg.xc, dtype=np.float64) * cfg.dx + off + cfg.x_start
+    y = np.arange(cfg.yc, dtype=np.float64) * cfg.dx + off + cfg.y_start
+    z = np.arange(cfg.zc, dtype=np.float64) * cfg.dx + off + cfg.z_start
+    return x, y, z
+
+
+def prim_to_cons_3d(cfg: Config, rho, vx, vy, vz, p) -> np.ndarray:
+    """prim_to_cons (Eos.hpp:81-99) for 3-D fields shaped [z][y][x], same grouping."""
+    q = np.zeros(cfg.shape)
+    q[0] = rho
+    q[1] = rho * vx
+    q[2] = rho * vy
+    q[3] = rho * vz
+    v2 = vx * vx
+    v2 = v2 + vy * vy
+    v2 = v2 + vz * vz
+    q[4] = p / (cfg.gamma - 1.0) + 0.5 * rho * v2
+    return q
+
+
+def blast_3d(cfg: Config, seed: int = 7, amp: float = 0.05, p_ratio: float = 10.0, radius: float = 0.2):
+    """Smooth seeded 3-D state (ghosts included): density, velocity and pressure are sums of low
+    Fourier modes with seeded phases, plus a spherical over-pressure of `p_ratio` within `radius` of
+    the domain centre."""
+    x, y, z = cell_centres_3d(cfg)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    L = cfg.x_dim
+    ph = 2 * np.pi * uniform01(seed, np.arange(12, dtype=np.uint64))
+
+    def mode(a, b, c, k):
+        return np.sin(2 * np.pi * (a * X + b * Y + c * Z) / L + ph[k])
+
+    rho = 1.0 + amp * (mode(1, 0, 1, 0) + mode(0, 1, 1, 1))
+    vx = amp * mode(0, 1, 1, 2)
+    vy = amp * mode(1, 0, 1, 3)
+    vz = amp * mode(1, 1, 0, 4)
+    cx = cfg.x_start + 0.5 * cfg.nx * cfg.dx
+    cy = cfg.y_start + 0.5 * cfg.ny * cfg.dx
+    cz = cfg.z_start + 0.5 * cfg.nz * cfg.dx
+    r2 = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2
+    p = (1.0 + amp * mode(1, 1, 1, 5)) * np.where(r2 < (radius * L) ** 2, p_ratio, 1.0)
+    return prim_to_cons_3d(cfg, rho, vx, vy, vz, p)
+
+
 # BASELINE configurations, with the reference's key names.
 def sod_config() -> Config:
     return Config(num_dim=2, nx=256, ny=256, ng=2, x_dim=1.0, bc=("zerograd",) * 4, reconstruction="muscl",

@@ -90,9 +90,9 @@ class OracleSim:
         c = OrcConfig()
         c.num_dim = cfg.num_dim
         c.num_tracers = getattr(cfg, "num_tracers", 0)
-        c.xc, c.yc, c.zc, c.ng = cfg.xc, cfg.yc, 1, cfg.ng
+        c.xc, c.yc, c.zc, c.ng = cfg.xc, cfg.yc, getattr(cfg, "zc", 1), cfg.ng
         c.dx = cfg.dx
-        c.loc[0], c.loc[1], c.loc[2] = cfg.x_start, cfg.y_start, 0.0
+        c.loc[0], c.loc[1], c.loc[2] = cfg.x_start, cfg.y_start, getattr(cfg, "z_start", 0.0)
         for k in range(6):
             c.bc[k] = BC[cfg.bc[k].lower()] if k < len(cfg.bc) else BC["wall"]
             for v in range(5):
