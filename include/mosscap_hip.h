@@ -189,6 +189,31 @@ int mh_eos_fields(mh_ctx* ctx, int32_t which, double avg_mass, int32_t ionisatio
 typedef int (*mh_user_bc_fn)(mh_ctx* ctx, void* user);
 int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user);
 
+/* ---- tabulated LTE hydrogen EOS: eos.type = tabulatedlteh (TabulatedLteH.hpp,
+ * Eos.cpp:66-88; the table file's variables, TabulatedLteH.hpp:30-55) ------- */
+typedef struct {
+    double log_eint_rho_start, log_eint_rho_step; /* log10(eint / rho) axis, eint in J m^-3 */
+    int32_t n_eint_rho;
+    double log_rho_start, log_rho_step;           /* log10(rho) axis */
+    int32_t n_rho;
+    const double* y_table;                        /* [n_rho][n_eint_rho] ionisation fraction */
+    const double* log_T_table;                    /* [n_rho][n_eint_rho] log10 temperature */
+} mh_lte_table;
+/* Installs the table (copied to the device) and sets the y and T fields to -1.
+ * From then on every stage starts with update_eos on its input
+ * (TabulatedLteH.hpp:120-161, called from compute_hydro_fluxes, Hydro.cpp:325-328):
+ * per cell, ghosts included, clamped bilinear interpolation of y and log10 T at
+ * (log10 eint/rho, log10 rho), and E += rho chi_H / m_H (y - y_prev) once y_prev
+ * is set. Stage 0's Q^n is the state before that update, as the reference copies
+ * Q_old first. A negative internal energy (the reference aborts) makes the step
+ * return MH_ERR_INVALID "eint negative". table = NULL removes the EOS. */
+int mh_set_tabulated_eos(mh_ctx* ctx, const mh_lte_table* table);
+/* update_eos on the current state, in place: what setup_sim does once before
+ * its first ghost fill (SetupModules.hpp:250-255). */
+int mh_tabulated_eos_update(mh_ctx* ctx);
+/* The y and T fields of the last update_eos, xc*yc*zc values each. */
+int mh_tabulated_eos_fields(mh_ctx* ctx, double* y, double* T);
+
 typedef struct {
     int32_t precision_form; /* MH_FORM_* */
     double* q;              /* FP64 state, or NULL */

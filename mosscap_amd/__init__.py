@@ -62,6 +62,12 @@ class MhStats(C.Structure):
                 ("prev_output_time", C.c_double)]
 
 
+class MhLteTable(C.Structure):
+    _fields_ = [("log_eint_rho_start", C.c_double), ("log_eint_rho_step", C.c_double), ("n_eint_rho", C.c_int32),
+                ("log_rho_start", C.c_double), ("log_rho_step", C.c_double), ("n_rho", C.c_int32),
+                ("y_table", C.c_void_p), ("log_T_table", C.c_void_p)]
+
+
 class MhDeviceView(C.Structure):
     _fields_ = [("precision_form", C.c_int32), ("q", C.c_void_p), ("q_hi", C.c_void_p), ("q_lo", C.c_void_p),
                 ("num_values", C.c_int64)]
@@ -113,6 +119,9 @@ _SIGNATURES = [
     ("mh_set_user_bc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_set_gravity", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double]),
     ("mh_eos_fields", C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("mh_set_tabulated_eos", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_tabulated_eos_fields", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_tabulated_eos_update", C.c_int, [C.c_void_p]),
     ("mh_device_state", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -254,6 +263,29 @@ class HydroContext:
         y, T = np.empty(n), np.empty(n)
         self._c(lib().mh_eos_fields(self._h, which, avg_mass, int(ionisation_e), _ptr(y), _ptr(T)))
         return y.reshape(shape), T.reshape(shape)
+
+    def set_tabulated_eos(self, table) -> None:
+        """eos.type = tabulatedlteh with `table` (a mosscap_amd.lte_table.LteTable), or None to remove it."""
+        if table is None:
+            self._c(lib().mh_set_tabulated_eos(self._h, None))
+            return
+        y = np.ascontiguousarray(table.y_table, dtype=np.float64)
+        lt = np.ascontiguousarray(table.log_T_table, dtype=np.float64)
+        assert y.shape == lt.shape == (table.n_rho, table.n_eint_rho)
+        t = MhLteTable(table.log_eint_rho_start, table.log_eint_rho_step, table.n_eint_rho, table.log_rho_start,
+                       table.log_rho_step, table.n_rho, y.ctypes.data, lt.ctypes.data)
+        self._c(lib().mh_set_tabulated_eos(self._h, C.byref(t)))
+
+    def tabulated_eos_update(self) -> None:
+        """update_eos on the current state in place (setup_sim's call before its first ghost fill)."""
+        self._c(lib().mh_tabulated_eos_update(self._h))
+
+    def tabulated_eos_fields(self):
+        """(y, T) kept by the tabulated EOS's last update_eos, shaped [z][y][x]."""
+        shape = (self.cfg.zc, self.cfg.yc, self.cfg.xc)
+        y, T = np.empty(shape), np.empty(shape)
+        self._c(lib().mh_tabulated_eos_fields(self._h, _ptr(y), _ptr(T)))
+        return y, T
 
     def set_gravity(self, g) -> None:
         """Register (gx, gy, gz) as a gravity source term, or remove it with None."""

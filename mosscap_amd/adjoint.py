@@ -82,3 +82,28 @@ def kh_adjoint_function():
             return torch.from_numpy(g).to(device=ctx.device, dtype=ctx.dtype), None
 
     return KHAdjoint
+
+
+def _main(argv=None) -> int:
+    """Forward + reverse sweep of the KH adjoint workload; prints the reverse sweep's device time
+    (the command the adjoint kernel-trace summaries are taken with)."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--n", type=int, default=512, help="grid cells per side")
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--every", type=int, default=20, help="checkpoint interval (steps)")
+    ap.add_argument("--solver", nargs="+", default=["rusanov", "hllc"])
+    a = ap.parse_args(argv)
+    for solver in a.solver:
+        prob = KHAdjointProblem(n=a.n, nsteps=a.steps, checkpoint_every=a.every, riemann_solver=solver)
+        c = 0.01 * (2 * problems.uniform01(17, np.arange(a.n * a.n)).reshape(a.n, a.n) - 1)
+        rho = prob.forward(c)
+        prob.backward(rho / np.linalg.norm(rho))
+        print(f"{solver}: {a.n}^2, {a.steps} steps, checkpoint every {a.every}: "
+              f"reverse sweep {prob.ctx.adjoint_backward_ms():.3f} ms", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())

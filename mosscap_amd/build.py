@@ -32,7 +32,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libmosscap_hip.so"
 HOST_LIB = PKG / "libmosscap_host.so"
 HOST_EXE = PKG / "mosscap_hip"
-HOST_UNITS = ["yaml_lite.cpp", "setup.cpp", "problems.cpp", "output_nc.cpp", "host_capi.cpp"]
+HOST_UNITS = ["yaml_lite.cpp", "setup.cpp", "problems.cpp", "output_nc.cpp", "table_nc.cpp", "host_capi.cpp"]
 ORACLE_SRC = ROOT / "oracle" / "mh_oracle.cpp"
 ORACLE_LIB = ROOT / "oracle" / "_build" / "libmh_oracle.so"
 

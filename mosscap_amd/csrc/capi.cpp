@@ -300,6 +300,78 @@ static hipError_t write_clock(mh_ctx* c, double max_cfl, double time, double max
     return hipMemcpyAsync(c->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream);
 }
 
+// The tabulated EOS's abort condition (TabulatedLteH.hpp:140-142) after a
+// synchronised step: a cell whose internal energy went negative.
+static int tab_check(mh_ctx* ctx) {
+    if (!ctx->tab_on) return MH_OK;
+    int neg = 0;
+    const hipError_t e = hipMemcpy(&neg, ctx->tab_neg, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return ctx->hip_fail(e, "tabulated EOS");
+    return neg ? ctx->fail(MH_ERR_INVALID, "eint negative") : MH_OK;
+}
+
+int mh_set_tabulated_eos(mh_ctx* ctx, const mh_lte_table* t) {
+    if (!ctx) return MH_ERR_INVALID;
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    for (double** p : {&ctx->tab_y, &ctx->tab_lT, &ctx->tab_yspace, &ctx->tab_Tspace}) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+    if (ctx->tab_neg) hipFree(ctx->tab_neg);
+    ctx->tab_neg = nullptr;
+    ctx->tab_on = 0;
+    ctx->drop_graphs();
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_set_tabulated_eos");
+    if (!t) return MH_OK;
+    if (t->n_eint_rho < 1 || t->n_rho < 1 || !t->y_table || !t->log_T_table || !(t->log_eint_rho_step > 0.0) ||
+        !(t->log_rho_step > 0.0) || (int64_t)t->n_eint_rho * t->n_rho > (int64_t)1 << 28)
+        return ctx->fail(MH_ERR_INVALID, "mh_set_tabulated_eos: bad table axes or null tables");
+    const size_t m = (size_t)t->n_eint_rho * t->n_rho, n = (size_t)ctx->ncells;
+    e = hipMalloc(&ctx->tab_y, m * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ctx->tab_lT, m * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ctx->tab_yspace, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ctx->tab_Tspace, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ctx->tab_neg, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(ctx->tab_y, t->y_table, m * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->tab_lT, t->log_T_table, m * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {  // y and T start at -1 (Eos.cpp:80)
+        const std::vector<double> minus_one(n, -1.0);
+        e = hipMemcpy(ctx->tab_yspace, minus_one.data(), n * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ctx->tab_Tspace, minus_one.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemset(ctx->tab_neg, 0, sizeof(int));
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_set_tabulated_eos");
+    ctx->tab = TabParams{t->log_eint_rho_start, t->log_eint_rho_step, t->n_eint_rho, t->log_rho_start, t->log_rho_step,
+                         t->n_rho};
+    ctx->tab_on = 1;
+    return MH_OK;
+}
+
+int mh_tabulated_eos_update(mh_ctx* ctx) {
+    if (!ctx) return MH_ERR_INVALID;
+    if (!ctx->tab_on) return ctx->fail(MH_ERR_INVALID, "no tabulated EOS is installed");
+    if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = ctx->tab_update(ctx->cur, ctx->cur);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_tabulated_eos_update");
+    return tab_check(ctx);
+}
+
+int mh_tabulated_eos_fields(mh_ctx* ctx, double* y, double* T) {
+    if (!ctx || !y || !T) return MH_ERR_INVALID;
+    if (!ctx->tab_on) return ctx->fail(MH_ERR_INVALID, "no tabulated EOS is installed");
+    const size_t n = (size_t)ctx->ncells;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(y, ctx->tab_yspace, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(T, ctx->tab_Tspace, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_tabulated_eos_fields");
+    return tab_check(ctx);
+}
+
 int mh_compute_dt(mh_ctx* ctx, double max_cfl, double time, double max_time, double prev_output_time,
                   double delta, double* dt_out) {
     if (!ctx || !dt_out) return MH_ERR_INVALID;
@@ -329,6 +401,7 @@ int mh_stage(mh_ctx* ctx, int32_t stage, double dt) {
     ctx->last_in = in;
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_stage");
+    if (const int rc = tab_check(ctx); rc != MH_OK) return rc;
     ctx->cur = out;
     ctx->next_stage = (stage + 1) % ns;
     if (ctx->next_stage == 0) {
@@ -348,6 +421,7 @@ int mh_time_step(mh_ctx* ctx, double dt) {
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_time_step");
     ctx->cur = nc;
+    if (const int rc = tab_check(ctx); rc != MH_OK) return rc;
     ctx->steps += 1;
     ctx->last_dt = dt;
     return MH_OK;
@@ -394,6 +468,7 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
     if (e == hipSuccess) e = hipMemcpyAsync(&h, ctx->clock, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_run");
+    if (const int rc = tab_check(ctx); rc != MH_OK) return rc;
     *time = h.time;
     *prev_output_time = h.prev_output_time;
     ctx->time = h.time;

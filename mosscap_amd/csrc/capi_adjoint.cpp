@@ -40,6 +40,7 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
                        double* dts_out) {
     if (!ctx || nsteps < 1 || checkpoint_every < 1) return MH_ERR_INVALID;
     if (ctx->fp32c() || ctx->nd != 2) return ctx->fail(MH_ERR_UNSUPPORTED, "the adjoint runs on FP64 2-D contexts");
+    if (ctx->tab_on) return ctx->fail(MH_ERR_UNSUPPORTED, "no adjoint for the tabulated EOS");
     const int32_t bcs[4] = {ctx->bdry.xs, ctx->bdry.xe, ctx->bdry.ys, ctx->bdry.ye};
     for (int32_t b : bcs) {
         if (b == (int32_t)BoundaryType::SymmetricOutflowDiode)

@@ -32,6 +32,13 @@ struct mh_ctx {
     double* eos_y = nullptr;         // analytic LTE EOS fields (mh_eos_fields)
     double* eos_T = nullptr;
     int* eos_flag = nullptr;
+    int32_t tab_on = 0;              // tabulated LTE EOS (mh_set_tabulated_eos): update_eos before each stage
+    TabParams tab{};
+    double* tab_y = nullptr;         // its device tables, [n_rho][n_eint_rho]
+    double* tab_lT = nullptr;
+    double* tab_yspace = nullptr;    // the y and T fields update_eos keeps (ghosts included)
+    double* tab_Tspace = nullptr;
+    int* tab_neg = nullptr;          // set when a cell's internal energy is negative
     int32_t gravity = 0;             // registered gravity source (mh_set_gravity)
     double grav[3] = {0.0, 0.0, 0.0};
     double* frx64 = nullptr;         // tracers: density fluxes of the stage (FP64 form)
@@ -190,11 +197,30 @@ struct mh_ctx {
         return launch_cfl_f64(nd, p64(k), (int)ncells, gamma, amax, stream);
     }
 
+    // Tabulated EOS update of buffer `in` into `dst` (update_eos at the start of
+    // compute_hydro_fluxes, Hydro.cpp:325-328).
+    hipError_t tab_update(int in, int dst) {
+        return fp32c() ? launch_tab_eos_f32c(nd, p32(in), p32(dst), (int)ncells, nvar, tab, tab_y, tab_lT, tab_yspace,
+                                             tab_Tspace, tab_neg, stream)
+                       : launch_tab_eos_f64(nd, p64(in), p64(dst), (int)ncells, nvar, tab, tab_y, tab_lT, tab_yspace,
+                                            tab_Tspace, tab_neg, stream);
+    }
+
     // One stage from buffer `in` into `out` with Q^n in `qn_k`, then ghost fill.
+    // With the tabulated EOS the stage reads its input after update_eos. The
+    // reference copies Q_old before the first stage's update_eos
+    // (TimeStepping.cpp:52, 129, 226), so for stage 0 (in == qn_k) the updated
+    // input goes to the third buffer, which no stage of the step has written
+    // yet, and Q^n keeps the values it had.
     hipError_t stage(int in, int qn_k, int out, int kind, int reduce) {
-        hipError_t e = timed(0, [&] { return stage_kernel(in, qn_k, out, kind, reduce); });
+        const int src = !tab_on ? in : (in == qn_k ? 3 - in - out : in);
+        hipError_t e = timed(0, [&] {
+            hipError_t r = tab_on ? tab_update(in, src) : hipSuccess;
+            if (r == hipSuccess) r = stage_kernel(src, qn_k, out, kind, reduce);
+            return r;
+        });
         if (e != hipSuccess) return e;
-        return ghost(in, out, reduce);
+        return ghost(src, out, reduce);
     }
 
     // 3-D march grid: no CFL fold inside (the 3-D ghost passes do not fold, so
@@ -302,8 +328,11 @@ struct mh_ctx {
         for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
         ev_pool.clear();
         for (void* t : {(void*)frx64, (void*)fry64, (void*)wtr64, (void*)frx32, (void*)fry32, (void*)wtr32,
-                        (void*)eos_y, (void*)eos_T, (void*)eos_flag, (void*)dz64, (void*)dz32})
+                        (void*)eos_y, (void*)eos_T, (void*)eos_flag, (void*)dz64, (void*)dz32, (void*)tab_y,
+                        (void*)tab_lT, (void*)tab_yspace, (void*)tab_Tspace, (void*)tab_neg})
             if (t) hipFree(t);
+        tab_y = tab_lT = tab_yspace = tab_Tspace = nullptr;
+        tab_neg = nullptr;
         dz64 = nullptr;
         dz32 = nullptr;
         frx64 = fry64 = wtr64 = nullptr;

@@ -101,6 +101,8 @@ struct FormF32c {
     __device__ static inline float2 negate(float2 v) { return make_float2(-v.x, -v.y); }
     __device__ static inline float2 from_double(double v) { return df::from_double(v); }
     __device__ static inline double value(float2 v) { return df::to_double(v); }
+    __device__ static inline const void* get_ptr(const Planes& a) { return a.hi; }
+    __device__ static inline float2 add_double(float2 v, double d) { return df::from_double(df::to_double(v) + d); }
 
     template <int ND, int NV, int IM>
     __device__ static inline void diode(float2 (&q)[NV], bool start) {

@@ -35,6 +35,8 @@ struct FormF64 {
     __device__ static inline double negate(double v) { return -v; }
     __device__ static inline double from_double(double v) { return v; }
     __device__ static inline double value(double v) { return v; }
+    __device__ static inline const void* get_ptr(const Planes& a) { return a.p; }
+    __device__ static inline double add_double(double v, double d) { return v + d; }
 
     // SymmetricOutflowDiode momentum clamp and energy correction,
     // Boundaries.hpp:96-106.

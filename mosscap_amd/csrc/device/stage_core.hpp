@@ -58,7 +58,10 @@ struct StageGrid {
 // interior z-plane per (strip, span, plane) workgroup; the z flux difference
 // of each cell comes from the z sweep (p.dz3) and is added after the y one,
 // as integrate_flux adds Fz after Fy (TimeStepping.cpp:22-30).
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int ND = 2>
+// Extras = false compiles out the gravity source and the tracers' density-flux
+// stores (their addresses and operands cost registers the plain hydro stage
+// needs); the launcher picks the instance from the pointers it is given.
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int ND = 2, bool Extras = true>
 __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Ptrs& p) {
     using T = typename Form::real;
     using Raw = typename Form::raw;
@@ -159,7 +162,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         commit_row(j0 + ORD, qb, qhb);
         recon_y(yl, yr);  // cell j0
         riemann_flux<RS, 1, ND>(eos, yRp, yl, Gyp);
-        if (ND == 2 && p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
+        if (Extras && ND == 2 && p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
         if (j0 < j1) fetch_row(j0 + 1 + ORD, qa, qha);
@@ -184,7 +187,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         T yl[NV], yr[NV], Gy[NV], dy[NV];
         recon_y(yl, yr);
         riemann_flux<RS, 1, ND>(eos, yRp, yl, Gy);
-        if (ND == 2 && p.fry != nullptr && out_lane) p.fry[(j + 1) * xc + c] = Gy[0];
+        if (Extras && ND == 2 && p.fry != nullptr && out_lane) p.fry[(j + 1) * xc + c] = Gy[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             dy[v] = Gyp[v] - Gy[v];
@@ -206,14 +209,14 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
         riemann_flux<RS, 0, ND>(eos, xrm, xl, Fx);
-        if (ND == 2 && p.frx != nullptr && lane >= 1 && c >= g.ng && c <= g.ng + g.nx) p.frx[j * xc + c] = Fx[0];
+        if (Extras && ND == 2 && p.frx != nullptr && lane >= 1 && c >= g.ng && c <= g.ng + g.nx) p.frx[j * xc + c] = Fx[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 
         if (out_lane) {
             Raw qo[NV];
             T srcv[NV];
-            if (p.gravity) {
+            if (Extras && p.gravity) {
                 Form::template gravity_src<ND>(p, qs_c, dt, srcv);
             } else {
 #pragma unroll

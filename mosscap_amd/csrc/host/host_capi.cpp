@@ -68,8 +68,18 @@ std::string summary(const Simulation& sim) {
       << ",\"precision_form\":" << sim.device.precision_form << ",\"span_rows\":" << sim.device.span_rows
       << ",\"rho_floor\":" << num(sim.device.rho_floor) << ",\"p_floor\":" << num(sim.device.p_floor)
       << ",\"gravity\":" << (sim.gravity ? "[" + num(sim.grav[0]) + "," + num(sim.grav[1]) + "," + num(sim.grav[2]) + "]"
-                                              : std::string("null"))
-      << "}";
+                                              : std::string("null"));
+    if (sim.eos.tabulated) {  // the table as read: axes and the sums of both tables
+        const LteTable& t = sim.eos.table;
+        double ys = 0.0, ts = 0.0;
+        for (double v : t.y_table) ys += v;
+        for (double v : t.log_T_table) ts += v;
+        o << ",\"lte_table\":{\"log_eint_rho_start\":" << num(t.log_eint_rho_start)
+          << ",\"log_eint_rho_step\":" << num(t.log_eint_rho_step) << ",\"n_eint_rho\":" << t.n_eint_rho
+          << ",\"log_rho_start\":" << num(t.log_rho_start) << ",\"log_rho_step\":" << num(t.log_rho_step)
+          << ",\"n_rho\":" << t.n_rho << ",\"y_sum\":" << num(ys) << ",\"log_T_sum\":" << num(ts) << "}";
+    }
+    o << "}";
     return o.str();
 }
 

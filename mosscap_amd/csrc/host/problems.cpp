@@ -1,6 +1,6 @@
 // Problem registry and the initial-condition generators of the workloads:
 // shock_tube_2d (ShockTube2d.cpp:10-49) and the seeded kh_band and sedov
-// setups. Every cell, ghosts included, is set on the host in FP64 with
+// (2-D and 3-D) setups. Every cell, ghosts included, is set on the host in FP64 with
 // prim_to_cons's grouping (Eos.hpp:81-99); the driver fills ghosts after.
 #include <array>
 #include <cmath>
@@ -38,6 +38,13 @@ void set_prim_2d(State& s, double gamma, int j, int i, double rho, double vx, do
     const double e_kin = 0.5 * rho * v2;
     const double e_int = p / (gamma - 1.0);
     s.q(3, 0, j, i) = e_int + e_kin;
+}
+
+// prim_to_cons (Eos.hpp:81-99) into cell (k, j, i) of a 3-D state at rest.
+void set_rest_3d(State& s, double gamma, int k, int j, int i, double rho, double p) {
+    s.q(0, k, j, i) = rho;
+    for (int a = 1; a <= 3; ++a) s.q(a, k, j, i) = rho * 0.0;
+    s.q(4, k, j, i) = p / (gamma - 1.0) + 0.5 * rho * 0.0;
 }
 
 void require_2d(const Simulation& sim, const char* name) {
@@ -113,11 +120,11 @@ void config_kh_band(Simulation& sim, const Config& config) {
 }
 
 // Sedov blast: rho = 1, v = 0, p = p_ambient, and the energy E spread
-// uniformly over the interior cells whose centre lies within r = radius_cells*dx
-// of the origin, p = (gamma - 1) E / (N dx^2) there (ghost cells inside the
-// radius get the same pressure).
+// uniformly over the N interior cells whose centre lies within
+// r = radius_cells*dx of the origin, p = (gamma - 1) E / (N dx^d) there in d =
+// 2 or 3 dimensions (ghost cells inside the radius get the same pressure).
 void config_sedov(Simulation& sim, const Config& config) {
-    require_2d(sim, "sedov");
+    if (sim.num_dim < 2) throw std::runtime_error("sedov only handles 2d and 3d problems");
     const double energy = get_or<double>(config, "problem.energy", 1.0);
     const double radius_cells = get_or<double>(config, "problem.radius_cells", 3.5);
     const double p_ambient = get_or<double>(config, "problem.p_ambient", 1e-5);
@@ -125,6 +132,23 @@ void config_sedov(Simulation& sim, const Config& config) {
     const double r = radius_cells * s.dx;
     const double r2 = r * r;
     const int g = s.sz.ng;
+    if (sim.num_dim == 3) {  // a sphere: p = (gamma - 1) E / (N dx^3)
+        auto inside3 = [&](int k, int j, int i) {
+            const double x = s.pos_x(i), y = s.pos_y(j), z = s.pos_z(k);
+            return (x * x + y * y + z * z) < r2;
+        };
+        long long n3 = 0;
+        for (int k = g; k < s.sz.zc - g; ++k)
+            for (int j = g; j < s.sz.yc - g; ++j)
+                for (int i = g; i < s.sz.xc - g; ++i) n3 += inside3(k, j, i) ? 1 : 0;
+        if (n3 == 0) throw std::runtime_error("sedov: no cell centre lies within problem.radius_cells");
+        const double p3 = (sim.eos.gamma - 1.0) * energy / (n3 * s.dx * s.dx * s.dx);
+        for (int k = 0; k < s.sz.zc; ++k)
+            for (int j = 0; j < s.sz.yc; ++j)
+                for (int i = 0; i < s.sz.xc; ++i)
+                    set_rest_3d(s, sim.eos.gamma, k, j, i, 1.0, inside3(k, j, i) ? p3 : p_ambient);
+        return;
+    }
     auto inside = [&](int j, int i) {
         const double x = s.pos_x(i), y = s.pos_y(j);
         return (x * x + y * y) < r2;

@@ -143,7 +143,8 @@ void setup_timestepper(Simulation& sim, const Config& config) {
     sim.max_steps = get_or<long long>(config, "timestep.max_steps", -1);
 }
 
-// Eos::init, Eos.cpp:12-63: the ideal gas and the analytic LTE hydrogen EOS.
+// Eos::init, Eos.cpp:12-88: the ideal gas and the analytic and tabulated LTE
+// hydrogen EOS (the table is read here, as TabulatedLteH::init does at setup).
 void setup_eos(Simulation& sim, const Config& config) {
     static const char* const EosTypeName[] = {"ideal", "analyticlteh", "tabulatedlteh", "dexpressure"};
     const std::string type = get_or<std::string>(config, "eos.type", "ideal");
@@ -157,9 +158,14 @@ void setup_eos(Simulation& sim, const Config& config) {
         sim.eos.gamma = get_or<double>(config, "eos.gamma", 5.0 / 3.0);
         sim.eos.include_ionisation_energy = get_or<bool>(config, "eos.include_ionisation_energy", false);
         sim.eos.is_constant = false;
+    } else if (t == 2) {
+        sim.eos.gamma = get_or<double>(config, "eos.gamma", 5.0 / 3.0);
+        const std::string table_path = get_or<std::string>(config, "eos.table_path", "mosscap_lte_h_tables.nc");
+        sim.eos.table = read_lte_table(table_path);
+        sim.eos.tabulated = true;
+        sim.eos.is_constant = false;
     } else {
-        throw std::runtime_error("eos.type " + type +
-                                 ": the tabulated and DexRT equations of state are not part of this build");
+        throw std::runtime_error("eos.type " + type + ": the DexRT equation of state is not part of this build");
     }
 }
 
@@ -170,6 +176,11 @@ static void update_eos_fields(Simulation& sim, int which) {
     const size_t n = (size_t)sim.state.cells();
     sim.eos.y_space.resize(n);
     sim.eos.T_space.resize(n);
+    if (sim.eos.tabulated) {  // the fields the device's update_eos kept
+        check(sim, mh_tabulated_eos_fields(sim.handle(), sim.eos.y_space.data(), sim.eos.T_space.data()),
+              "update_eos");
+        return;
+    }
     check(sim, mh_eos_fields(sim.handle(), which, sim.eos.avg_mass, sim.eos.include_ionisation_energy ? 1 : 0,
                              sim.eos.y_space.data(), sim.eos.T_space.data()),
           "update_eos");
@@ -326,6 +337,13 @@ void attach_device(Simulation& sim, bool write_initial_output) {
     }
     sim.ctx = std::shared_ptr<mh_ctx>(raw, [](mh_ctx* c) { mh_destroy(c); });
     check(sim, mh_upload(raw, sim.state.Q.data()), "upload");
+    if (sim.eos.tabulated) {
+        const LteTable& t = sim.eos.table;
+        const mh_lte_table mt{t.log_eint_rho_start, t.log_eint_rho_step, t.n_eint_rho, t.log_rho_start,
+                              t.log_rho_step,      t.n_rho,             t.y_table.data(), t.log_T_table.data()};
+        check(sim, mh_set_tabulated_eos(raw, &mt), "tabulated EOS");
+        check(sim, mh_tabulated_eos_update(raw), "update_eos");
+    }
     // setup_sim runs update_eos on the problem's state before filling ghosts (SetupModules.hpp:250-255)
     update_eos_fields(sim, 0);
 

@@ -74,12 +74,25 @@ struct NumericalSchemes {
 
 // Eos (Eos.hpp:27-47): ideal gas, or the analytic LTE hydrogen EOS whose
 // ionisation fraction and temperature fields are written with the output.
+// The tabulated LTE hydrogen EOS table (TabulatedLteH.hpp:10-28), read from
+// eos.table_path by read_lte_table (table_nc.cpp).
+struct LteTable {
+    double log_eint_rho_start = 0.0, log_eint_rho_step = 0.0;
+    int n_eint_rho = 0;
+    double log_rho_start = 0.0, log_rho_step = 0.0;
+    int n_rho = 0;
+    std::vector<double> y_table, log_T_table;  // [n_rho][n_eint_rho]
+};
+LteTable read_lte_table(const std::string& path);
+
 struct Eos {
     double gamma = 1.4;
     bool is_constant = true;
     double avg_mass = 1.0;
     bool include_ionisation_energy = false;
-    std::vector<double> y_space, T_space;  // analytic LTE fields of the last update_eos
+    bool tabulated = false;                // eos.type = tabulatedlteh
+    LteTable table;
+    std::vector<double> y_space, T_space;  // LTE fields of the last update_eos
 };
 
 // OutputConfig (Output.hpp), read by setup_output (SetupModules.hpp:179-196).

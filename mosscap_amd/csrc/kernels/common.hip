@@ -204,6 +204,31 @@ hipError_t launch_ghost_f32c(int nd, const GhostGrid& g, FormF32c::Planes in, Fo
                              unsigned long long* amax, hipStream_t st) {
     return launch_ghost_impl<FormF32c>(nd, g, in, out, amax, st);
 }
+template <class Form>
+hipError_t launch_tab_eos_impl(int nd, typename Form::Planes in, typename Form::Planes out, int ncells, int nvar,
+                               const TabParams& t, const double* ytab, const double* ltab, double* y_space,
+                               double* T_space, int* neg, hipStream_t st) {
+    if (ncells < 1 || nvar < 2 + nd || t.ex_num < 1 || t.r_num < 1 || !ytab || !ltab || !y_space || !T_space || !neg)
+        return hipErrorInvalidValue;
+    const dim3 grid(blocks_for(ncells)), block(kBlock);
+    if (nd == 1) hipLaunchKernelGGL((tab_eos_kernel<Form, 1>), grid, block, 0, st, in, out, ncells, nvar, t, ytab, ltab, y_space, T_space, neg);
+    else if (nd == 2) hipLaunchKernelGGL((tab_eos_kernel<Form, 2>), grid, block, 0, st, in, out, ncells, nvar, t, ytab, ltab, y_space, T_space, neg);
+    else if (nd == 3) hipLaunchKernelGGL((tab_eos_kernel<Form, 3>), grid, block, 0, st, in, out, ncells, nvar, t, ytab, ltab, y_space, T_space, neg);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_tab_eos_f64(int nd, FormF64::Planes in, FormF64::Planes out, int ncells, int nvar,
+                              const TabParams& t, const double* ytab, const double* ltab, double* y_space,
+                              double* T_space, int* neg, hipStream_t st) {
+    return launch_tab_eos_impl<FormF64>(nd, in, out, ncells, nvar, t, ytab, ltab, y_space, T_space, neg, st);
+}
+hipError_t launch_tab_eos_f32c(int nd, FormF32c::Planes in, FormF32c::Planes out, int ncells, int nvar,
+                               const TabParams& t, const double* ytab, const double* ltab, double* y_space,
+                               double* T_space, int* neg, hipStream_t st) {
+    return launch_tab_eos_impl<FormF32c>(nd, in, out, ncells, nvar, t, ytab, ltab, y_space, T_space, neg, st);
+}
+
 hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, hipStream_t st) {
     return launch_ghost3d_impl<FormF64>(g, q, st);
 }

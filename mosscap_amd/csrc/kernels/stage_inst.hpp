@@ -13,10 +13,10 @@
 
 namespace mh {
 
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, bool Extras>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MH_STAGE_WAVES_PER_EU)))
 void stage2d_kernel(StageGrid g, typename Form::Ptrs p) {
-    stage2d_march<Form, R, SL, RS>(g, p);
+    stage2d_march<Form, R, SL, RS, 2, Extras>(g, p);
 }
 
 struct Stage1dGrid {
@@ -51,11 +51,12 @@ using Stage1dFn = void (*)(Stage1dGrid, typename Form::Ptrs);
 
 template <class Form, Reconstruction R, SlopeLimiter SL>
 struct SolverPick {
+    template <bool Extras>
     static Stage2dFn<Form> two(RiemannSolver rs) {
         switch (rs) {
-            case RiemannSolver::Rusanov: return stage2d_kernel<Form, R, SL, RiemannSolver::Rusanov>;
-            case RiemannSolver::Hll: return stage2d_kernel<Form, R, SL, RiemannSolver::Hll>;
-            default: return stage2d_kernel<Form, R, SL, RiemannSolver::Hllc>;
+            case RiemannSolver::Rusanov: return stage2d_kernel<Form, R, SL, RiemannSolver::Rusanov, Extras>;
+            case RiemannSolver::Hll: return stage2d_kernel<Form, R, SL, RiemannSolver::Hll, Extras>;
+            default: return stage2d_kernel<Form, R, SL, RiemannSolver::Hllc, Extras>;
         }
     }
     static Stage1dFn<Form> one(RiemannSolver rs) {
@@ -93,7 +94,12 @@ hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typen
         g.yc != g.ny + 2 * g.ng || g.span_h < 1 || g.strips != (g.nx + kStripOut - 1) / kStripOut) {
         return hipErrorInvalidValue;
     }
-    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() { return SolverPick<Form, R, SL>::two(s.solver); };
+    // the plain instance unless a gravity source or tracer flux stores are asked for
+    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr;
+    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() {
+        return extras ? SolverPick<Form, R, SL>::template two<true>(s.solver)
+                      : SolverPick<Form, R, SL>::template two<false>(s.solver);
+    };
     Stage2dFn<Form> fn = pick_scheme<Form>(s, pick);
     const int spans = (g.ny + g.span_h - 1) / g.span_h;
     hipLaunchKernelGGL(fn, dim3((unsigned)(g.strips * spans)), dim3(kWave), 0, st, g, p);

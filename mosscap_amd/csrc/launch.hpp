@@ -13,6 +13,7 @@
 #include "device/stage3d.hpp"
 #include "device/tracer.hpp"
 #include "device/lte_eos.hpp"
+#include "device/lte_table.hpp"
 
 namespace mh {
 
@@ -65,6 +66,15 @@ hipError_t launch_lte_eos_f64(int nd, FormF64::Planes q, int ncells, const LtePa
                               int* flipped, hipStream_t st);
 hipError_t launch_lte_eos_f32c(int nd, FormF32c::Planes q, int ncells, const LteParams& p, double* y, double* T,
                                int* flipped, hipStream_t st);
+
+// Tabulated LTE EOS update_eos over every cell of `in` into `out` (same
+// buffer, or a copy of all nvar planes with E updated).
+hipError_t launch_tab_eos_f64(int nd, FormF64::Planes in, FormF64::Planes out, int ncells, int nvar,
+                              const TabParams& t, const double* ytab, const double* ltab, double* y_space,
+                              double* T_space, int* neg, hipStream_t st);
+hipError_t launch_tab_eos_f32c(int nd, FormF32c::Planes in, FormF32c::Planes out, int ncells, int nvar,
+                               const TabParams& t, const double* ytab, const double* ltab, double* y_space,
+                               double* T_space, int* neg, hipStream_t st);
 
 hipError_t launch_clock_finalize(Clock* c, int advance, hipStream_t st);
 hipError_t launch_clock_set_dt(Clock* c, double dt, hipStream_t st);

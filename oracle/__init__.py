@@ -68,6 +68,11 @@ def lib() -> C.CDLL:
             "orc_reconstruct": (None, [C.c_int, C.c_int, i64, vp, vp, vp]),
             "orc_riemann": (None, [C.c_int, C.c_int, C.c_int, d, i64, vp, vp, vp]),
             "orc_lte_eos": (C.c_int, [C.c_int, i64, vp, d, d, C.c_int, vp, vp]),
+            "orc_set_tabulated_eos": (None, [vp, d, d, C.c_int32, d, d, C.c_int32, vp, vp]),
+            "orc_tabulated_fields": (C.c_int, [vp, vp, vp]),
+            "orc_set_y_space": (None, [vp, vp]),
+            "orc_update_eos": (None, [vp]),
+            "orc_tab_sample": (None, [d, d, C.c_int32, d, d, C.c_int32, vp, vp, i64, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -133,6 +138,28 @@ class OracleSim:
         w = np.empty(self.shape)
         lib().orc_get_w(self._h, _p(w))
         return w
+
+    def set_tabulated_eos(self, t) -> None:
+        """eos.type = tabulatedlteh with table t (attributes log_eint_rho_start/step, n_eint_rho,
+        log_rho_start/step, n_rho, y_table and log_T_table [n_rho][n_eint_rho])."""
+        self._tab = (np.ascontiguousarray(t.y_table, dtype=np.float64),
+                     np.ascontiguousarray(t.log_T_table, dtype=np.float64))
+        lib().orc_set_tabulated_eos(self._h, t.log_eint_rho_start, t.log_eint_rho_step, t.n_eint_rho,
+                                    t.log_rho_start, t.log_rho_step, t.n_rho, _p(self._tab[0]), _p(self._tab[1]))
+
+    def tabulated_fields(self):
+        """(y, T, eint_negative) of the last update_eos, shaped [z][y][x]."""
+        shape = self.shape[1:]
+        y, T = np.empty(shape), np.empty(shape)
+        neg = lib().orc_tabulated_fields(self._h, _p(y), _p(T))
+        return y, T, bool(neg)
+
+    def set_y_space(self, y: np.ndarray) -> None:
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        lib().orc_set_y_space(self._h, _p(y))
+
+    def update_eos(self) -> None:
+        lib().orc_update_eos(self._h)
 
     def fill_bcs(self) -> None:
         lib().orc_fill_bcs(self._h)
@@ -213,6 +240,18 @@ def riemann(solver, axis, nd, gamma, wl, wr):
     f = np.empty_like(a)
     lib().orc_riemann(SOLVER[solver], axis, nd, gamma, a.shape[0], _p(a), _p(b), _p(f))
     return f
+
+
+def tab_sample(t, log_eint_rho, log_rho):
+    """TabulatedLteH::sample on arrays of points: (y, log10 T)."""
+    a = np.ascontiguousarray(log_eint_rho, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(log_rho, dtype=np.float64).ravel()
+    yt = np.ascontiguousarray(t.y_table, dtype=np.float64)
+    lt = np.ascontiguousarray(t.log_T_table, dtype=np.float64)
+    y, lT = np.empty(a.size), np.empty(a.size)
+    lib().orc_tab_sample(t.log_eint_rho_start, t.log_eint_rho_step, t.n_eint_rho, t.log_rho_start, t.log_rho_step,
+                         t.n_rho, _p(yt), _p(lt), a.size, _p(a), _p(b), _p(y), _p(lT))
+    return y, lT
 
 
 def lte_eos(nd, q, gamma, avg_mass=1.0, ionisation_e=False):

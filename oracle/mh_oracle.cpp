@@ -295,12 +295,26 @@ typedef struct {
 
 }  // extern "C"
 
+// TabulatedLteH (TabulatedLteH.hpp:10-28): two log-spaced axes and the y and
+// log10 T tables, [n_rho][n_eint_rho].
+struct orc_tab {
+    double ex_start, ex_step;  // log10(eint / rho)
+    int32_t ex_num;
+    double r_start, r_step;    // log10(rho)
+    int32_t r_num;
+    std::vector<double> y, log_T;
+};
+
 struct orc_sim {
     orc_config c;
     int nvar;
     size_t cells;
     std::vector<double> Q, W, RL, RR, Fx, Fy, Fz, S, Q_old;
     int64_t current_step = 0;
+    bool tab_on = false;               // eos.type = tabulatedlteh: update_eos before each stage
+    orc_tab tab;
+    std::vector<double> y_space, T_space;
+    int eint_negative = 0;             // the reference aborts ("eint negative")
 
     double* at(std::vector<double>& a, int v, int k, int j, int i) {
         return &a[(((size_t)v * c.zc + k) * c.yc + j) * c.xc + i];
@@ -402,8 +416,89 @@ void compute_flux(orc_sim& s, int axis) {
     }
 }
 
-// compute_hydro_fluxes, Hydro.cpp:325-343 (no update_eos for the ideal EOS)
+// TabulatedLteH::sample (TabulatedLteH.hpp:57-118): bilinear interpolation
+// in (log10 eint/rho, log10 rho), clamped to the table's edge outside it. The
+// floor-based index is formed only inside the table, so a NaN coordinate (a
+// state that already went NaN, where the reference's int conversion is
+// undefined) takes the first edge instead of indexing out of the table; every
+// other input gives the reference's indices.
+void tab_sample(const orc_tab& t, double log_eint_rho, double log_rho, double& y, double& log_T) {
+    const double frac_x = (log_eint_rho - t.ex_start) / t.ex_step;
+    int ix;
+    int ixp;
+    double tx, txp;
+    const int max_x = t.ex_num - 1;
+    const int max_y = t.r_num - 1;
+    if (!(frac_x >= 0.0) || frac_x >= max_x) {  // outside the table (or NaN): its edge
+        ix = frac_x >= max_x ? max_x : 0;
+        ixp = ix;
+        tx = 1.0;
+        txp = 0.0;
+    } else {
+        ix = int(std::floor(frac_x));
+        ixp = ix + 1;
+        txp = frac_x - ix;
+        tx = 1.0 - txp;
+    }
+    const double frac_y = (log_rho - t.r_start) / t.r_step;
+    int iy;
+    int iyp;
+    double ty, typ;
+    if (!(frac_y >= 0.0) || frac_y >= max_y) {
+        iy = frac_y >= max_y ? max_y : 0;
+        iyp = iy;
+        ty = 1.0;
+        typ = 0.0;
+    } else {
+        iy = int(std::floor(frac_y));
+        iyp = iy + 1;
+        typ = frac_y - iy;
+        ty = 1.0 - typ;
+    }
+    auto at = [&](const std::vector<double>& a, int r, int x) { return a[(size_t)r * t.ex_num + x]; };
+    y = ty * (tx * at(t.y, iy, ix) + txp * at(t.y, iy, ixp)) + typ * (tx * at(t.y, iyp, ix) + txp * at(t.y, iyp, ixp));
+    log_T = ty * (tx * at(t.log_T, iy, ix) + txp * at(t.log_T, iy, ixp)) +
+            typ * (tx * at(t.log_T, iyp, ix) + txp * at(t.log_T, iyp, ixp));
+}
+
+// TabulatedLteH::update_eos (TabulatedLteH.hpp:120-161): per cell (ghosts
+// included) the table's y and T for the cell's density and internal energy,
+// and E += rho chi_H / m_H (y - y_prev) once y_prev has been set (the fields
+// start at -1, Eos.cpp:80).
+void update_eos_tab(orc_sim& s) {
+    constexpr double h_mass = 1.6737830080950003e-27, chi_H = 2.178710282685096e-18;
+    const int nd = s.c.num_dim;
+    const size_t n = s.cells;
+    double* Q = s.Q.data();
+    int neg = 0;
+#pragma omp parallel for schedule(static) reduction(| : neg)
+    for (size_t c = 0; c < n; ++c) {
+        double mom2_sum = sq(Q[n + c]);
+        if (nd > 1) mom2_sum += sq(Q[2 * n + c]);
+        if (nd > 2) mom2_sum += sq(Q[3 * n + c]);
+        const double rho = Q[c];
+        const double e_kin = 0.5 * mom2_sum / rho;
+        const double eint = Q[(1 + nd) * n + c] - e_kin;
+        if (eint < 0.0) {
+            neg = 1;
+            continue;
+        }
+        const double log_rho = std::log10(rho);
+        const double log_eint_rho = std::log10(eint) - log_rho;
+        double y, log_T;
+        tab_sample(s.tab, log_eint_rho, log_rho, y, log_T);
+        const double prev_y = s.y_space[c];
+        s.y_space[c] = y;
+        const double delta_eint = (prev_y < 0.0) ? 0.0 : rho * (chi_H / h_mass) * (y - prev_y);
+        s.T_space[c] = std::pow(10.0, log_T);
+        Q[(1 + nd) * n + c] += delta_eint;
+    }
+    s.eint_negative |= neg;
+}
+
+// compute_hydro_fluxes, Hydro.cpp:325-343 (update_eos only for the tabulated EOS here)
 void compute_hydro_fluxes(orc_sim& s) {
+    if (s.tab_on) update_eos_tab(s);
     global_cons_to_prim(s);
     compute_recon(s, 0);
     compute_flux(s, 0);
@@ -732,6 +827,43 @@ int orc_lte_eos(int nd, int64_t ncells, const double* Q, double gamma, double av
         T_out[c] = temp;
     }
     return flipped;
+}
+
+// ---- tabulated LTE hydrogen EOS (TabulatedLteH.hpp, Eos.cpp:66-88) ----------
+// Installs the table (y and log10 T, [n_rho][n_eint_rho]) and sets the y and
+// T fields to -1; from then on every stage starts with update_eos.
+void orc_set_tabulated_eos(orc_sim* s, double ex_start, double ex_step, int32_t ex_num, double r_start,
+                           double r_step, int32_t r_num, const double* y_table, const double* log_T_table) {
+    s->tab_on = true;
+    s->tab.ex_start = ex_start;
+    s->tab.ex_step = ex_step;
+    s->tab.ex_num = ex_num;
+    s->tab.r_start = r_start;
+    s->tab.r_step = r_step;
+    s->tab.r_num = r_num;
+    const size_t m = (size_t)ex_num * r_num;
+    s->tab.y.assign(y_table, y_table + m);
+    s->tab.log_T.assign(log_T_table, log_T_table + m);
+    s->y_space.assign(s->cells, -1.0);
+    s->T_space.assign(s->cells, -1.0);
+    s->eint_negative = 0;
+}
+// The fields of the last update_eos; returns 1 if a cell had eint < 0.
+int orc_tabulated_fields(const orc_sim* s, double* y, double* T) {
+    std::memcpy(y, s->y_space.data(), s->cells * sizeof(double));
+    std::memcpy(T, s->T_space.data(), s->cells * sizeof(double));
+    return s->eint_negative;
+}
+void orc_set_y_space(orc_sim* s, const double* y) { std::memcpy(s->y_space.data(), y, s->cells * sizeof(double)); }
+void orc_update_eos(orc_sim* s) { update_eos_tab(*s); }
+void orc_tab_sample(double ex_start, double ex_step, int32_t ex_num, double r_start, double r_step, int32_t r_num,
+                    const double* y_table, const double* log_T_table, int64_t n, const double* log_eint_rho,
+                    const double* log_rho, double* y, double* log_T) {
+    orc_tab t{ex_start, ex_step, ex_num, r_start, r_step, r_num, {}, {}};
+    const size_t m = (size_t)ex_num * r_num;
+    t.y.assign(y_table, y_table + m);
+    t.log_T.assign(log_T_table, log_T_table + m);
+    for (int64_t i = 0; i < n; ++i) tab_sample(t, log_eint_rho[i], log_rho[i], y[i], log_T[i]);
 }
 
 // ---- single-functor entry points for the unit tests ------------------------

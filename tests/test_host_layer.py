@@ -170,7 +170,8 @@ def test_constant_bc_mapping_1d(H):
     (MIN + "scheme:\n  riemann_solver: roe\n", r'Unable to find associated key in ".*RiemannSolver" for "roe".'),
     (MIN + "boundary:\n  xs: outflow\n", r'for "outflow"'),
     (MIN + "boundary:\n  xs: constant\n  xs_const: xxx\n", r"Provide constant boundary values for xs_const"),
-    (MIN + "eos:\n  type: tabulatedlteh\n", r"tabulated and DexRT equations of state are not part"),
+    (MIN + "eos:\n  type: tabulatedlteh\n", r"eos.table_path mosscap_lte_h_tables.nc: cannot open the file"),
+    (MIN + "eos:\n  type: dexpressure\n", r"DexRT equation of state is not part"),
     (MIN + "output:\n  variables:\n    fluxes: true\n", r"fluxes / source"),
     ("simulation:\n  num_dim: 2\n", r"No method for circular_explosion found."),
     (MIN + "grid:\n  dx: -1.0\n", r"grid.dx cannot be negative."),
@@ -315,3 +316,91 @@ def test_analytic_lte_eos_setup_and_output(H, tmp_path):
     with netcdf(tmp_path / "lte.nc", "r", mmap=False) as f:
         assert f.variables["ion_frac"].shape == (2, 1, 262, 262)
         assert f.variables["T"].shape == (2, 1, 262, 262)
+
+
+SEDOV_3D = ("simulation:\n  num_dim: 3\ngrid:\n  x: 20\n  y: 18\n  z: 16\n  x_dim: 2.0\n  x_start: -1.0\n"
+            "  y_start: -0.9\n  z_start: -0.8\nboundary:\n  xs: zerograd\n  xe: zerograd\n  ys: wall\n  ye: wall\n"
+            "  zs: periodic\n  ze: periodic\nscheme:\n  reconstruction: ppm\n  riemann_solver: hll\n"
+            "timestep:\n  max_cfl: 0.3\n  max_steps: 4\noutput:\n  name: sedov3d\nproblem:\n  name: sedov\n"
+            "  radius_cells: 2.5\n")
+
+
+def test_sedov_3d_setup_and_energy(H):
+    s = H.setup_summary(SEDOV_3D)
+    assert (s["num_dim"], s["xc"], s["yc"], s["zc"]) == (3, 26, 24, 22)
+    assert s["bc"][4:] == [1, 1] and s["loc"] == [-1.0, -0.9, -0.8]
+    q = H.initial_state(SEDOV_3D)
+    assert q.shape == (5, 22, 24, 26)
+    g, dx = 3, 0.1
+    assert not np.any(q[1:4])
+    e = q[4, g:-g, g:-g, g:-g]
+    ambient = 1e-5 / (1.4 - 1.0)
+    hot = e > ambient
+    assert 20 < hot.sum() < 120  # a ball of radius 2.5 cells
+    assert np.isclose(((e[hot] - ambient) * dx ** 3).sum() + hot.sum() * ambient * dx ** 3, 1.0, rtol=1e-12)
+    x = np.arange(26) * dx - 2.5 * dx - 1.0
+    y = np.arange(24) * dx - 2.5 * dx - 0.9
+    z = np.arange(22) * dx - 2.5 * dx - 0.8
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    np.testing.assert_array_equal(q[4] > ambient, X * X + Y * Y + Z * Z < (2.5 * dx) ** 2)
+
+
+def test_netcdf_3d_layout(H, tmp_path):
+    netcdf = pytest.importorskip("scipy.io").netcdf_file
+    H.write_initial_output(SEDOV_3D, tmp_path)
+    q0 = H.initial_state(SEDOV_3D)
+    with netcdf(tmp_path / "sedov3d.nc", "r", mmap=False) as f:
+        assert (f.dimensions["x"], f.dimensions["y"], f.dimensions["z"], f.dimensions["var"]) == (26, 24, 22, 5)
+        a = f._attributes
+        assert (a["irho"], a["imx"], a["imy"], a["imz"], a["iene"]) == (0, 1, 2, 3, 4)
+        np.testing.assert_array_equal(f.variables["z"][:], np.arange(22) * 0.1 + (-(3 - 0.5) * 0.1) - 0.8)
+        np.testing.assert_array_equal(f.variables["Q"][0], q0)
+
+
+# ---- tabulated LTE EOS table files -------------------------------------------------
+def test_lte_table_reader_formats(H, tmp_path):
+    """The C++ reader takes CDF-1 and CDF-2 files with int or double scalars and float or double
+    tables (the generator writes CDF-2), refuses HDF5 and reports missing variables."""
+    from scipy.io import netcdf_file
+    from mosscap_amd.lte_table import generate, write_nc
+    t = generate(12, 6)
+    good = tmp_path / "t2.nc"
+    write_nc(t, good)
+    base = MIN + "eos:\n  type: tabulatedlteh\n  table_path: {}\n"
+    s = H.setup_summary(base.format(good))
+    assert s["gamma"] == 5.0 / 3.0
+
+    def check(lt, yt, tt):
+        assert (lt["n_eint_rho"], lt["n_rho"]) == (t.n_eint_rho, t.n_rho)
+        assert (lt["log_eint_rho_start"], lt["log_eint_rho_step"]) == (t.log_eint_rho_start, t.log_eint_rho_step)
+        assert (lt["log_rho_start"], lt["log_rho_step"]) == (t.log_rho_start, t.log_rho_step)
+        assert lt["y_sum"] == sum(float(v) for v in yt.ravel())
+        assert lt["log_T_sum"] == sum(float(v) for v in tt.ravel())
+
+    check(s["lte_table"], t.y_table, t.log_T_table)
+    # CDF-1, float tables, double counts
+    p1 = tmp_path / "t1.nc"
+    with netcdf_file(str(p1), "w", version=1) as f:
+        f.createDimension("a", t.n_rho)
+        f.createDimension("b", t.n_eint_rho)
+        for nm, val in (("log_eint_rho_start", t.log_eint_rho_start), ("log_eint_rho_step", t.log_eint_rho_step),
+                        ("n_eint_rho", float(t.n_eint_rho)), ("log_rho_start", t.log_rho_start),
+                        ("log_rho_step", t.log_rho_step), ("n_rho", float(t.n_rho))):
+            v = f.createVariable(nm, "d", ())
+            v[()] = val
+        for nm, arr in (("y_table", t.y_table), ("log_T_table", t.log_T_table)):
+            v = f.createVariable(nm, "f", ("a", "b"))
+            v[:] = arr.astype(np.float32)
+    s1 = H.setup_summary(base.format(p1))
+    check(s1["lte_table"], t.y_table.astype(np.float32).astype(np.float64),
+          t.log_T_table.astype(np.float32).astype(np.float64))
+    h5 = tmp_path / "t.h5"
+    h5.write_bytes(b"\x89HDF\r\n\x1a\n" + bytes(64))
+    with pytest.raises(H.HostError, match="HDF5"):
+        H.setup_summary(base.format(h5))
+    p3 = tmp_path / "t3.nc"
+    with netcdf_file(str(p3), "w", version=2) as f:
+        v = f.createVariable("log_eint_rho_start", "d", ())
+        v[()] = 1.0
+    with pytest.raises(H.HostError, match="no variable log_eint_rho_step"):
+        H.setup_summary(base.format(p3))
