@@ -291,6 +291,43 @@ def bench_sedov(a):
     return 0
 
 
+def bench_blast3d(a):
+    """3-D blast (WENO5-Z + HLLC + SSPRK3, periodic, CFL 0.4) on an n^3 grid: FP64 and FP32C
+    throughput of graph-replayed steps, FP32C-vs-FP64 rel L1 after the timed steps, and the
+    algorithmic bandwidth (5 variables x 8 B x (2 x 3 + 2) = 320 B per cell-step)."""
+    import mosscap_amd as M
+    from mosscap_amd import Config, problems
+    n = a.n3
+    cfg = Config(num_dim=3, nx=n, ny=n, nz=n, ng=3, x_dim=1.0, bc=("periodic",) * 6, reconstruction="weno5z",
+                 riemann_solver="hllc", time_stepper="ssprk3", max_cfl=0.4, output_delta=1e30, max_time=1e30)
+    q0 = problems.blast_3d(cfg, seed=3)
+    res, states = {}, {}
+    for form in (M.FP64, M.FP32C):
+        el, tm, tune = time_form(M, cfg, q0, form, a.steps, a.warmup)
+        res[form] = (el, tm, tune)
+        with M.HydroContext(cfg, precision_form=form) as ctx:
+            ctx.upload(q0)
+            ctx.fill_bcs()
+            ctx.run(a.steps)
+            states[form] = ctx.download()
+    cells = n ** 3
+    out = {"metric": "3-D blast Mcell-updates/s", "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
+           "config": {"workload": f"3-D blast {n}^3 periodic, ng=3, WENO5-Z + HLLC + SSPRK3, CFL 0.4, "
+                                  "hipGraph-replayed steps (z sweep + per-plane x-y march)"}}
+    for form, name in ((M.FP64, "fp64"), (M.FP32C, "fp32c")):
+        el, tm, tune = res[form]
+        v = cells * a.steps / el / 1e6
+        out[name] = {"value": round(v, 1), "ms_per_step": round(el / a.steps * 1e3, 4),
+                     "algorithmic_GBs": round(v * 1e6 * 320 / 1e9, 1), "span_rows": tune["span_rows"] if tune else None,
+                     "kernel_ms_per_step": {k: round(getattr(tm, k) / a.steps, 4) for k in ("stage_ms", "ghost_ms", "clock_ms")}}
+    g = cfg.ng
+    a32, a64 = states[M.FP32C][:, g:-g, g:-g, g:-g], states[M.FP64][:, g:-g, g:-g, g:-g]
+    out["rel_l1_fp32c_vs_fp64"] = [float(f"{r:.3e}") for r in
+                                   (np.abs(a32 - a64).reshape(5, -1).sum(1) / np.abs(a64).reshape(5, -1).sum(1))]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,12 +346,15 @@ def main(argv=None):
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="summarise rocprofv3 --pmc CSVs into the traffic JSON (CPU only) and exit")
     ap.add_argument("--copy-bytes", type=int, default=512 << 20)
-    ap.add_argument("--workload", choices=["kh", "batch", "sedov"], default="kh",
-                    help="kh: the headline line; batch / sedov: the other BASELINE workloads")
+    ap.add_argument("--workload", choices=["kh", "batch", "sedov", "blast3d"], default="kh",
+                    help="kh: the headline line; batch / sedov: the other workloads; blast3d: the 3-D path")
+    ap.add_argument("--n3", type=int, default=256, help="cells per side of the 3-D blast")
     ap.add_argument("--batch-steps", type=int, default=500)
     a = ap.parse_args(argv)
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "blast3d":
+        return bench_blast3d(a)
     if a.workload == "sedov":
         if a.n == 2048:
             a.n = 4096

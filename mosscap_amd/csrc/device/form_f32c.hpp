@@ -17,6 +17,7 @@
 #pragma once
 
 #include "functors.hpp"
+#include "wave.hpp"
 
 namespace mh {
 
@@ -93,6 +94,19 @@ struct FormF32c {
     };
 
     __device__ static inline float2 get(const Planes& a, int idx) { return make_float2(a.hi[idx], a.lo[idx]); }
+    // Buffer-resource access for the plain march (see FormF64::Buf).
+    struct Buf {
+        __amdgpu_buffer_rsrc_t hi, lo;
+    };
+    static constexpr uint32_t kElem = 4;
+    __device__ static inline Buf buf(const Planes& a) { return Buf{make_rsrc(a.hi), make_rsrc(a.lo)}; }
+    __device__ static inline float2 bget(const Buf& b, uint32_t voff, uint32_t s_elem) {
+        return make_float2(buf_load_f32(b.hi, voff, s_elem * kElem), buf_load_f32(b.lo, voff, s_elem * kElem));
+    }
+    __device__ static inline void bput(const Buf& b, uint32_t voff, uint32_t s_elem, float2 v) {
+        buf_store_f32(v.x, b.hi, voff, s_elem * kElem);
+        buf_store_f32(v.y, b.lo, voff, s_elem * kElem);
+    }
     __device__ static inline void put(const Planes& a, int idx, float2 v) {
         a.hi[idx] = v.x;
         a.lo[idx] = v.y;

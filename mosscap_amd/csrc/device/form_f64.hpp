@@ -3,6 +3,7 @@
 #pragma once
 
 #include "functors.hpp"
+#include "wave.hpp"
 
 namespace mh {
 
@@ -30,6 +31,20 @@ struct FormF64 {
     };
 
     __device__ static inline double get(const Planes& a, int idx) { return a.p[idx]; }
+    // Buffer-resource access for the plain march: the wave-uniform part of
+    // the element index goes in a scalar offset, the lane's part in a vector
+    // offset that stays fixed over the march (no per-load 64-bit address math).
+    struct Buf {
+        __amdgpu_buffer_rsrc_t p;
+    };
+    static constexpr uint32_t kElem = 8;
+    __device__ static inline Buf buf(const Planes& a) { return Buf{make_rsrc(a.p)}; }
+    __device__ static inline double bget(const Buf& b, uint32_t voff, uint32_t s_elem) {
+        return buf_load_f64(b.p, voff, s_elem * kElem);
+    }
+    __device__ static inline void bput(const Buf& b, uint32_t voff, uint32_t s_elem, double v) {
+        buf_store_f64(v, b.p, voff, s_elem * kElem);
+    }
     __device__ static inline void put(const Planes& a, int idx, double v) { a.p[idx] = v; }
     __device__ static inline bool needs_qn(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 5; }
     __device__ static inline double negate(double v) { return -v; }

@@ -96,6 +96,18 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int h_entry = lane < ORD ? lane : kWave + lane;
     const int h_col = min(i0 - 1 - ORD + h_entry, g.xc - 1);
 
+    // The plain instance addresses Q through buffer resources: scalar offsets
+    // for the wave-uniform row/plane/variable part, fixed per-lane vector
+    // offsets (the launcher keeps every byte offset below 2^31).
+    constexpr bool kBuf = !Extras;
+    const auto bqs = Form::buf(p.qs);
+    const auto bqn = Form::buf(p.qn);
+    const auto bout = Form::buf(p.out);
+    constexpr uint32_t ES = Form::kElem;
+    auto load_qs = [&](int col, int uniform) -> Raw {
+        if constexpr (kBuf) return Form::bget(bqs, (uint32_t)col * ES, (uint32_t)uniform);
+        else return Form::get(p.qs, uniform + col);
+    };
     const EosT<T> eos = make_eos<T>((double)p.gamma);
     const T dt = (T)p.dt[0];
     const T dtdx = dt / (T)p.dx;
@@ -105,7 +117,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const bool out_lane = lane >= 1 && lane <= kStripOut && c < g.ng + g.nx;
     unsigned int floor_hits = 0;
 
-    auto commit_row = [&](int r, const Raw (&q)[NV], const Raw (&qh)[NV]) {
+    auto commit_row = [&](int r, const Raw (&q)[NV]) {
         const int slot = r & (RING - 1);
         T w[NV];
         const bool hit = Form::template to_prim_nd<ND>(p, q, w);
@@ -120,20 +132,19 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) win[WIN - 1][v] = w[v];
         if (ORD > 0 && halo_lane) {
+            Raw qh[NV];  // halo columns: loaded at commit, not prefetched (registers)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qh[v] = load_qs(h_col, v * plane + pk + r * xc);
             T wh[NV];
             Form::template to_prim_nd<ND>(p, qh, wh);
 #pragma unroll
             for (int v = 0; v < NV; ++v) ring[slot][v][h_entry] = wh[v];
         }
     };
-    auto fetch_row = [&](int r, Raw (&q)[NV], Raw (&qh)[NV]) {
+    auto fetch_row = [&](int r, Raw (&q)[NV]) {
         const int base = pk + r * xc;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) q[v] = Form::get(p.qs, v * plane + base + c_ld);
-        if (ORD > 0 && halo_lane) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qh[v] = Form::get(p.qs, v * plane + base + h_col);
-        }
+        for (int v = 0; v < NV; ++v) q[v] = load_qs(c_ld, v * plane + base);
     };
     auto recon_y = [&](T (&wl)[NV], T (&wr)[NV]) {
 #pragma unroll
@@ -146,42 +157,34 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     };
 
     // ---- prologue: rows j0-1-ORD .. j0+ORD; y faces up to j0 ------------------
-    Raw qa[NV], qha[NV];
+    Raw qa[NV];
     T yRp[NV], Gyp[NV];
     {
-        Raw qb[NV], qhb[NV];
+        Raw qb[NV];
         for (int r = j0 - 1 - ORD; r <= j0 - 1 + ORD; ++r) {
-            fetch_row(r, qb, qhb);
-            commit_row(r, qb, qhb);
+            fetch_row(r, qb);
+            commit_row(r, qb);
         }
         T yl[NV], yr[NV];
         recon_y(yl, yr);  // cell j0-1
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
-        fetch_row(j0 + ORD, qb, qhb);
-        commit_row(j0 + ORD, qb, qhb);
+        fetch_row(j0 + ORD, qb);
+        commit_row(j0 + ORD, qb);
         recon_y(yl, yr);  // cell j0
         riemann_flux<RS, 1, ND>(eos, yRp, yl, Gyp);
         if (Extras && ND == 2 && p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
-        if (j0 < j1) fetch_row(j0 + 1 + ORD, qa, qha);
+        if (j0 < j1) fetch_row(j0 + 1 + ORD, qa);
     }
 
     double amax = 0.0;
 
     for (int j = j0; j < j1; ++j) {
-        // Stage-combine operands for this row, issued early.
-        Raw qs_c[NV], qn_c[NV];
-        const int cell = pk + j * xc + c_ld;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            qs_c[v] = Form::get(p.qs, v * plane + cell);
-            qn_c[v] = Form::needs_qn(g.kind) ? Form::get(p.qn, v * plane + cell) : qs_c[v];
-        }
         wave_lds_sync();  // earlier ring reads complete before the slot is reused
-        commit_row(j + 1 + ORD, qa, qha);
-        if (j + 1 < j1) fetch_row(j + 2 + ORD, qa, qha);  // prefetch the next row
+        commit_row(j + 1 + ORD, qa);
+        if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
 
         // y sweep: cell j+1, face j+1
         T yl[NV], yr[NV], Gy[NV], dy[NV];
@@ -214,6 +217,20 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 
         if (out_lane) {
+            // stage-combine operands, loaded here rather than at the top of the
+            // row: fewer registers live across the sweeps (higher occupancy)
+            Raw qs_c[NV], qn_c[NV];
+            const int row = pk + j * xc;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                qs_c[v] = load_qs(c_ld, v * plane + row);
+                if constexpr (kBuf) {
+                    qn_c[v] = Form::needs_qn(g.kind) ? Form::bget(bqn, (uint32_t)c_ld * ES, (uint32_t)(v * plane + row))
+                                                     : qs_c[v];
+                } else {
+                    qn_c[v] = Form::needs_qn(g.kind) ? Form::get(p.qn, v * plane + row + c_ld) : qs_c[v];
+                }
+            }
             Raw qo[NV];
             T srcv[NV];
             if (Extras && p.gravity) {
@@ -230,7 +247,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 if constexpr (ND > 2) q_update += p.dz3[v * plane + pk + j * xc + c];
                 q_update *= dtdx;
                 qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v]);
-                Form::put(p.out, v * plane + pk + j * xc + c, qo[v]);
+                if constexpr (kBuf) Form::bput(bout, (uint32_t)c * ES, (uint32_t)(v * plane + row), qo[v]);
+                else Form::put(p.out, v * plane + row + c, qo[v]);
             }
             if (g.reduce_dt) {
                 const double a = Form::template signal_speed_nd<ND>((double)p.gamma, qo);

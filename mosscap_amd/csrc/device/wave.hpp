@@ -77,4 +77,24 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
     return xcd < rem ? xcd * (per + 1) + pos : rem * (per + 1) + (xcd - rem) * per + pos;
 }
 
+// Raw buffer resources (gfx9 V#: stride 0, num_records 0xFFFFFFFF, 32-bit
+// data format word). Callers keep scalar offset + vector offset below 2^31
+// bytes; the resource's range check is not relied on.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
+}
+__device__ inline float buf_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ inline void buf_store_f32(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+__device__ inline double buf_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ inline void buf_store_f64(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    using V2 = __attribute__((ext_vector_type(2))) uint32_t;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, (int)voff, (int)soff, 0);
+}
+
 }  // namespace mh

@@ -94,8 +94,11 @@ hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typen
         g.yc != g.ny + 2 * g.ng || g.span_h < 1 || g.strips != (g.nx + kStripOut - 1) / kStripOut) {
         return hipErrorInvalidValue;
     }
-    // the plain instance unless a gravity source or tracer flux stores are asked for
-    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr;
+    // the plain instance (buffer-resource addressing, no source or tracer
+    // stores) unless a gravity source or tracer flux stores are asked for, or
+    // the state's byte offsets do not fit the buffer offsets' 31 bits
+    const bool big = (int64_t)4 * g.xc * g.yc * Form::kElem >= ((int64_t)1 << 31);
+    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr || big;
     auto pick = [&]<Reconstruction R, SlopeLimiter SL>() {
         return extras ? SolverPick<Form, R, SL>::template two<true>(s.solver)
                       : SolverPick<Form, R, SL>::template two<false>(s.solver);

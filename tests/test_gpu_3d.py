@@ -29,7 +29,8 @@ SHAPES = [(40, 36, 28), (70, 20, 9)]  # the second spans two strips in x
 
 def _cfg(case, shape):
     nx, ny, nz = shape
-    cfg = Config(num_dim=3, nx=nx, ny=ny, nz=nz, ng=3, x_dim=nx / 32.0, max_cfl=0.4, **CASES[case])
+    # CFL 0.3: the unsplit 3-D update adds three directions' Courant numbers
+    cfg = Config(num_dim=3, nx=nx, ny=ny, nz=nz, ng=3, x_dim=nx / 32.0, max_cfl=0.3, **CASES[case])
     if "constant" in cfg.bc:
         cfg.bc_const = tuple((1.0, 0.05 * k, -0.03, 0.02, 2.6) for k in range(6))
     return cfg
@@ -77,9 +78,10 @@ def test_fp64_3d_step_entry_points(orc):
         sim.time_step(dt)
         np.testing.assert_array_equal(ctx.download(), sim.get_q())
         sim.begin_step()
-        for s in range(2):  # rk2: two stages
+        for s in range(2):  # rk2: two stages, each ending with the ghost fill
             ctx.stage(s, dt)
             sim.stage(s, dt)
+            sim.fill_bcs()
         np.testing.assert_array_equal(ctx.download(), sim.get_q())
 
 
@@ -127,9 +129,13 @@ def test_lte_fields_3d(orc):
         ctx.upload(q)
         y, T = ctx.eos_fields(which=0)
     yo, To = orc.lte_eos(3, q, cfg.gamma)
-    close = np.abs(T.ravel() - To.ravel()) <= 0.05 + 1e-12 * To.ravel()
-    assert close.mean() > 0.999
-    np.testing.assert_allclose(y.ravel()[close], yo.ravel()[close], rtol=1e-9, atol=1e-15)
+    # the bisection's path is the same wherever T is bitwise equal; elsewhere a last-bit
+    # difference of the device's pow/exp moved one decision by a step of the bisection
+    T, To, y, yo = T.ravel(), To.ravel(), y.ravel(), yo.ravel()
+    same = T == To
+    assert same.mean() > 0.99
+    np.testing.assert_allclose(y[same], yo[same], rtol=1e-12, atol=1e-15)
+    assert np.all(np.abs(T - To) <= 0.3 * np.maximum(1.0, 1e-4 * To))
 
 
 @pytest.mark.parametrize("what", ["userfn", "tracers", "adjoint"])
