@@ -98,6 +98,9 @@ struct FormF32c {
     struct Buf {
         __amdgpu_buffer_rsrc_t hi, lo;
     };
+    // the march loads Q^(s) and Q^n for the combine at the top of the row
+    // (their latency hides behind the sweeps; 4 waves per SIMD either way)
+    static constexpr bool kEarlyOperands = true;
     static constexpr uint32_t kElem = 4;
     __device__ static inline Buf buf(const Planes& a) { return Buf{make_rsrc(a.hi), make_rsrc(a.lo)}; }
     __device__ static inline float2 bget(const Buf& b, uint32_t voff, uint32_t s_elem) {

@@ -37,6 +37,9 @@ struct FormF64 {
     struct Buf {
         __amdgpu_buffer_rsrc_t p;
     };
+    // the march loads Q^(s) and Q^n for the combine just before it (fewer
+    // registers live across the sweeps: 2 waves per SIMD instead of 1)
+    static constexpr bool kEarlyOperands = false;
     static constexpr uint32_t kElem = 8;
     __device__ static inline Buf buf(const Planes& a) { return Buf{make_rsrc(a.p)}; }
     __device__ static inline double bget(const Buf& b, uint32_t voff, uint32_t s_elem) {

@@ -182,6 +182,24 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     double amax = 0.0;
 
     for (int j = j0; j < j1; ++j) {
+        // Stage-combine operands of this row: loaded here or just before the
+        // combine (Form::kEarlyOperands; latency hiding against registers).
+        const int row = pk + j * xc;
+        Raw qs_c[NV], qn_c[NV];
+        auto load_operands = [&] {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                qs_c[v] = load_qs(c_ld, v * plane + row);
+                if constexpr (kBuf) {
+                    qn_c[v] = Form::needs_qn(g.kind)
+                                  ? Form::bget(bqn, (uint32_t)c_ld * ES, (uint32_t)(v * plane + row))
+                                  : qs_c[v];
+                } else {
+                    qn_c[v] = Form::needs_qn(g.kind) ? Form::get(p.qn, v * plane + row + c_ld) : qs_c[v];
+                }
+            }
+        };
+        if constexpr (Form::kEarlyOperands) load_operands();
         wave_lds_sync();  // earlier ring reads complete before the slot is reused
         commit_row(j + 1 + ORD, qa);
         if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
@@ -217,20 +235,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 
         if (out_lane) {
-            // stage-combine operands, loaded here rather than at the top of the
-            // row: fewer registers live across the sweeps (higher occupancy)
-            Raw qs_c[NV], qn_c[NV];
-            const int row = pk + j * xc;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                qs_c[v] = load_qs(c_ld, v * plane + row);
-                if constexpr (kBuf) {
-                    qn_c[v] = Form::needs_qn(g.kind) ? Form::bget(bqn, (uint32_t)c_ld * ES, (uint32_t)(v * plane + row))
-                                                     : qs_c[v];
-                } else {
-                    qn_c[v] = Form::needs_qn(g.kind) ? Form::get(p.qn, v * plane + row + c_ld) : qs_c[v];
-                }
-            }
+            if constexpr (!Form::kEarlyOperands) load_operands();
             Raw qo[NV];
             T srcv[NV];
             if (Extras && p.gravity) {

@@ -1,10 +1,13 @@
 // Fused stage kernels of the compensated FP32 precision form. Built with
 // -ffp-contract=fast: multiply-adds may fuse, except inside the error-free
 // transforms of the stage combine (form_f32c.hpp pins those with a pragma).
+// The plain instance fits 128 VGPRs (4 waves per SIMD); the instance with
+// the gravity source and tracer stores needs more (3 waves).
 #ifndef MH_STAGE_WAVES_F32C
-#define MH_STAGE_WAVES_F32C 3
+#define MH_STAGE_WAVES_F32C 4
 #endif
 #define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F32C
+#define MH_STAGE_WAVES_EXTRAS 3
 #include "stage_inst.hpp"
 
 namespace mh {
