@@ -147,7 +147,8 @@ int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, doub
  * wavefront in 2-D; 0-terminated list, NULL = built-in candidates) with HIP
  * events, keeps the fastest for this context and returns it in *best_span
  * (and the per-candidate milliseconds per step in ms_out when non-NULL). The
- * state is restored afterwards. */
+ * state, the tabulated EOS's fields and the floor-hit count are restored
+ * afterwards; a user_bc hook runs during the trials like in any step. */
 int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* best_span, double* ms_out);
 
 int mh_get_stats(mh_ctx* ctx, mh_stats* out);

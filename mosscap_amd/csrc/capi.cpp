@@ -539,6 +539,23 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
         return r;
     };
     if (e == hipSuccess) e = stash(false);
+    // what the trial steps change besides the state: the tabulated EOS's fields,
+    // the floor-hit count, the stage-input record
+    std::vector<double> keep_y, keep_T;
+    unsigned int keep_hits = 0;
+    if (e == hipSuccess && ctx->tab_on) {
+        keep_y.resize((size_t)ctx->ncells);
+        keep_T.resize((size_t)ctx->ncells);
+        e = hipMemcpyAsync(keep_y.data(), ctx->tab_yspace, keep_y.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(keep_T.data(), ctx->tab_Tspace, keep_T.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&keep_hits, ctx->floor_hits, sizeof(keep_hits), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    const int old_last_in = ctx->last_in;
     const int old_span = ctx->span_h;
     const int64_t old_steps = ctx->steps;
     double best = 1e300;
@@ -574,8 +591,19 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
         }
         ctx->cur = start;
         if (e == hipSuccess) e = stash(true);
+        if (e == hipSuccess && ctx->tab_on) {
+            e = hipMemcpyAsync(ctx->tab_yspace, keep_y.data(), keep_y.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(ctx->tab_Tspace, keep_T.data(), keep_T.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipMemsetAsync(ctx->tab_neg, 0, sizeof(int), ctx->stream);
+        }
     }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ctx->floor_hits, &keep_hits, sizeof(keep_hits), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx->last_in = old_last_in;
     if (a) hipEventDestroy(a);
     if (b) hipEventDestroy(b);
     if (keep_a) hipFree(keep_a);

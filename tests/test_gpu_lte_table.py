@@ -102,6 +102,50 @@ def test_step_entry_points_and_graph_loop_agree():
     np.testing.assert_array_equal(out[0], out[2])
 
 
+def test_with_tracers_and_gravity(orc):
+    """The tabulated EOS's stage-0 copy carries the tracer planes; the gravity/tracer stage
+    instance reads the updated input."""
+    cfg = _cfg(num_tracers=2, gravity=(-30.0, 12.0, 0.0))
+    q0 = _state(cfg)
+    q0[4] = 0.3 * q0[0]
+    q0[5] = q0[0] * (q0[1] > 0)
+    sim = orc.OracleSim(cfg)
+    sim.set_tabulated_eos(TABLE)
+    sim.set_q(q0)
+    sim.update_eos()
+    sim.fill_bcs()
+    t_o, _, _ = sim.run(max_steps=6)
+    with M.HydroContext(cfg) as ctx:
+        ctx.set_tabulated_eos(TABLE)
+        ctx.upload(q0)
+        ctx.tabulated_eos_update()
+        ctx.fill_bcs()
+        t_g, _ = ctx.run(6)
+        got = ctx.download()
+    assert abs(t_g - t_o) <= 1e-12 * t_o
+    _close(got, sim.get_q())
+
+
+def test_autotune_leaves_the_eos_fields():
+    """mh_autotune's trial steps run update_eos; it restores the fields with the state, so a run
+    after it equals a run without it."""
+    cfg = _cfg()
+    q0 = _state(cfg)
+    out = []
+    for tune in (False, True):
+        with M.HydroContext(cfg) as ctx:
+            ctx.set_tabulated_eos(TABLE)
+            ctx.upload(q0)
+            ctx.tabulated_eos_update()
+            ctx.fill_bcs()
+            if tune:
+                ctx.autotune(steps=2, candidates=[8, 16])
+            ctx.run(4, use_graph=False)
+            out.append((ctx.download(), ctx.tabulated_eos_fields()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1][0], out[1][1][0])
+
+
 def test_fp32c_close_to_fp64():
     cfg = _cfg()
     q0 = _state(cfg)
