@@ -1,7 +1,9 @@
 // 3-D stage kernels (z sweep + per-plane x-y march) of the compensated FP32
 // form, built with -ffp-contract=fast like the 2-D unit. Five variables per
-// cell need more registers than the 2-D march: two waves per SIMD.
-#define MH_STAGE_WAVES_PER_EU 2
+// cell need more registers than the 2-D march: three waves per SIMD for the
+// plain instance (160 VGPRs), two for the one with the gravity source.
+#define MH_STAGE_WAVES_PER_EU 3
+#define MH_STAGE_WAVES_EXTRAS 2
 #define MH_STAGE3D_INSTANCES 1
 #include "stage_inst.hpp"
 

@@ -128,10 +128,11 @@ hipError_t launch_stage1d_impl(const Schemes& s, int kind, int reduce_dt, int xc
 }
 
 #ifdef MH_STAGE3D_INSTANCES
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MH_STAGE_WAVES_PER_EU)))
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, bool Extras>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Extras ? MH_STAGE_WAVES_EXTRAS : MH_STAGE_WAVES_PER_EU)))
 void stage3d_kernel(StageGrid g, typename Form::Ptrs p) {
-    stage2d_march<Form, R, SL, RS, 3>(g, p);
+    stage2d_march<Form, R, SL, RS, 3, Extras>(g, p);
 }
 
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
@@ -155,11 +156,21 @@ hipError_t launch_stage3d_impl(const Schemes& s, const StageGrid& g, const typen
             default: return zsweep_kernel<Form, R, SL, RiemannSolver::Hllc>;
         }
     };
+    // plain instance (buffer addressing, no source) unless gravity is on or a
+    // byte offset could pass 2^31
+    const bool big = (int64_t)5 * g.xc * g.yc * g.zc * Form::kElem >= ((int64_t)1 << 31);
+    const bool extras = p.gravity != 0 || big;
     auto mpick = [&]<Reconstruction R, SlopeLimiter SL>() -> Stage2dFn<Form> {
         switch (s.solver) {
-            case RiemannSolver::Rusanov: return stage3d_kernel<Form, R, SL, RiemannSolver::Rusanov>;
-            case RiemannSolver::Hll: return stage3d_kernel<Form, R, SL, RiemannSolver::Hll>;
-            default: return stage3d_kernel<Form, R, SL, RiemannSolver::Hllc>;
+            case RiemannSolver::Rusanov:
+                return extras ? stage3d_kernel<Form, R, SL, RiemannSolver::Rusanov, true>
+                              : stage3d_kernel<Form, R, SL, RiemannSolver::Rusanov, false>;
+            case RiemannSolver::Hll:
+                return extras ? stage3d_kernel<Form, R, SL, RiemannSolver::Hll, true>
+                              : stage3d_kernel<Form, R, SL, RiemannSolver::Hll, false>;
+            default:
+                return extras ? stage3d_kernel<Form, R, SL, RiemannSolver::Hllc, true>
+                              : stage3d_kernel<Form, R, SL, RiemannSolver::Hllc, false>;
         }
     };
     // z sweep: chunks of at most 32 planes per thread, columns of the interior across the grid's x
