@@ -369,7 +369,7 @@ class HydroContext:
 
     def autotune(self, steps: int = 3, candidates=None):
         """Pick the span height (rows per wavefront) by timing; returns (best, {span: ms/step})."""
-        cand = np.array(list(candidates or [4, 8, 12, 16, 24, 32, 48, 64]) + [0], dtype=np.int32)
+        cand = np.array(list(candidates or [4, 8, 12, 16, 18, 20, 24, 28, 32, 48, 64]) + [0], dtype=np.int32)
         ms = np.zeros(len(cand))
         best = C.c_int32()
         self._c(lib().mh_autotune(self._h, steps, _ptr(cand), C.byref(best), _ptr(ms)))

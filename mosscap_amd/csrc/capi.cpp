@@ -106,10 +106,17 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
     if (c->opt.span_rows > 0) {
         c->span_h = c->opt.span_rows;
     } else {
-        // enough wavefronts to fill 256 CUs several times over, long spans
-        // where the grid allows (the y halo costs 2R+2 rows per span)
+        // One round of wavefronts: the shortest span (>= 8 rows; the y halo
+        // costs 2R+2 rows per span) whose wavefronts all fit 1024 SIMDs at 4
+        // waves each, so none waits for a second round; 32 rows when even
+        // that needs several rounds.
         c->span_h = 32;
-        while (c->span_h > 8 && (int64_t)c->strips * ((ny + c->span_h - 1) / c->span_h) * nz < 4096) c->span_h /= 2;
+        for (int h = 8; h <= 32; ++h) {
+            if ((int64_t)c->strips * ((ny + h - 1) / h) * nz <= 4096) {
+                c->span_h = h;
+                break;
+            }
+        }
     }
 
     hipError_t e = hipGetDevice(&c->device);
@@ -515,7 +522,7 @@ int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, doub
 int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* best_span, double* ms_out) {
     if (!ctx || !best_span || steps < 1) return MH_ERR_INVALID;
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
-    static const int32_t kDefault[] = {4, 8, 12, 16, 24, 32, 48, 64, 0};
+    static const int32_t kDefault[] = {4, 8, 12, 16, 18, 20, 24, 28, 32, 48, 64, 0};
     const int32_t* cand = candidates ? candidates : kDefault;
     hipError_t e = hipSetDevice(ctx->device);
     // keep the state in a scratch copy and restore it after each trial

@@ -150,8 +150,10 @@ struct mh_ctx {
         return g;
     }
 
-    Ghost3Grid ghost3_grid() const {
+    Ghost3Grid ghost3_grid(int reduce) const {
         Ghost3Grid g{};
+        g.gamma = gamma;
+        g.reduce_dt = reduce;
         g.xc = sz.xc;
         g.yc = sz.yc;
         g.zc = sz.zc;
@@ -167,13 +169,13 @@ struct mh_ctx {
     }
 
     // Ghost fill of `out` after a stage from `in`. In 3-D the three axis
-    // passes run on `out` alone and the CFL fold is a separate pass.
+    // passes run on `out` alone.
     hipError_t ghost(int in, int out, int reduce) {
         unsigned long long* amax = &clock->amax_bits;
         hipError_t e = timed(1, [&] {
             if (nd == 3)
-                return fp32c() ? launch_ghost3d_f32c(ghost3_grid(), p32(out), stream)
-                               : launch_ghost3d_f64(ghost3_grid(), p64(out), stream);
+                return fp32c() ? launch_ghost3d_f32c(ghost3_grid(reduce), p32(out), amax, stream)
+                               : launch_ghost3d_f64(ghost3_grid(reduce), p64(out), amax, stream);
             hipError_t r = fp32c() ? launch_ghost_f32c(nd, ghost_grid(reduce), p32(in), p32(out), amax, stream)
                                    : launch_ghost_f64(nd, ghost_grid(reduce), p64(in), p64(out), amax, stream);
             if (r != hipSuccess || ntr == 0) return r;
@@ -223,10 +225,9 @@ struct mh_ctx {
         return ghost(src, out, reduce);
     }
 
-    // 3-D march grid: no CFL fold inside (the 3-D ghost passes do not fold, so
-    // the whole grid is reduced after the last stage instead)
-    StageGrid grid3(int kind) const {
-        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, 0};
+    // 3-D march grid (the interior fold of the CFL max with `reduce`)
+    StageGrid grid3(int kind, int reduce) const {
+        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
         g.zc = sz.zc;
         g.nz = sz.zc - 2 * sz.ng;
         return g;
@@ -245,7 +246,7 @@ struct mh_ctx {
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
             } else if (nd == 3) {
                 p.gz = grav[2];
-                e = launch_stage3d_f32c(sch, grid3(kind), p, dz32, stream);
+                e = launch_stage3d_f32c(sch, grid3(kind, reduce), p, dz32, stream);
             } else {
                 e = launch_stage1d_f32c(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
@@ -258,7 +259,7 @@ struct mh_ctx {
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
             } else if (nd == 3) {
                 p.gz = grav[2];
-                e = launch_stage3d_f64(sch, grid3(kind), p, dz64, stream);
+                e = launch_stage3d_f64(sch, grid3(kind, reduce), p, dz64, stream);
             } else {
                 e = launch_stage1d_f64(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
@@ -290,10 +291,10 @@ struct mh_ctx {
     }
 
     // All stages of one step starting from `cur`; returns the new cur.
-    // With a user_bc hook, and in 3-D, the reduction is a separate pass at the end.
+    // With a user_bc hook the reduction is a separate pass after the hook.
     int enqueue_step(int k, bool reduce_last, hipError_t& e) {
         const int ns = num_stages(sch.stepper);
-        const bool fused = reduce_last && !user_fn && nd != 3;
+        const bool fused = reduce_last && !user_fn;
         int in = k;
         for (int s = 0; s < ns; ++s) {
             last_in = in;

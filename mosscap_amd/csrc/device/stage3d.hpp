@@ -97,11 +97,17 @@ struct Ghost3Grid {
     int32_t xc, yc, zc, ng;
     int32_t bc[6];          // xs, xe, ys, ye, zs, ze (BoundaryType)
     double bconst[6][5];    // Constant-boundary conserved values per face
+    double gamma;
+    int32_t reduce_dt;      // fold the cells this pass leaves final into the CFL max
 };
 
 // One pass of fill_one_bc_impl<Axis, 3> (Boundaries.hpp:11-115) over buffer q.
+// With reduce_dt the pass folds c_s + |v| of the ghost cells no later pass
+// rewrites (x: interior j and k; y: interior k; z: all), so with the stage's
+// interior fold every cell enters the CFL max exactly once.
 template <class Form, int Axis>
-__global__ __launch_bounds__(256) void ghost_pass3d_kernel(Ghost3Grid g, typename Form::Planes q) {
+__global__ __launch_bounds__(256) void ghost_pass3d_kernel(Ghost3Grid g, typename Form::Planes q,
+                                                           unsigned long long* amax_bits) {
     using Raw = typename Form::raw;
     constexpr int ND = 3;
     constexpr int NV = 5;
@@ -110,27 +116,38 @@ __global__ __launch_bounds__(256) void ghost_pass3d_kernel(Ghost3Grid g, typenam
     ld[Axis] = 2 * g.ng;
     const int n = ld[0] * ld[1] * ld[2];
     const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= n) return;
-    int c[3] = {t % ld[0], (t / ld[0]) % ld[1], t / (ld[0] * ld[1])};
-    const int pencil = c[Axis];
-    const bool start = pencil < g.ng;
-    c[Axis] = start ? pencil : (dims[Axis] - 1) - (pencil - g.ng);
-    const int face = 2 * Axis + (start ? 0 : 1);
-    const BoundaryType b = (BoundaryType)g.bc[face];
-    if (b == BoundaryType::UserFn) return;  // left to the caller (3-D contexts refuse it)
-    const int vs = g.xc * g.yc * g.zc;
-    Raw v[NV];
-    if (b != BoundaryType::Constant) {
-        int s[3] = {c[0], c[1], c[2]};
-        s[Axis] = ghost_source(c[Axis], dims[Axis], g.ng, b);
-        const int src = (s[2] * g.yc + s[1]) * g.xc + s[0];
+    double a = 0.0;
+    if (t < n) {
+        int c[3] = {t % ld[0], (t / ld[0]) % ld[1], t / (ld[0] * ld[1])};
+        const int pencil = c[Axis];
+        const bool start = pencil < g.ng;
+        c[Axis] = start ? pencil : (dims[Axis] - 1) - (pencil - g.ng);
+        const int face = 2 * Axis + (start ? 0 : 1);
+        const BoundaryType b = (BoundaryType)g.bc[face];
+        if (b != BoundaryType::UserFn) {  // userfn: left to the caller (3-D contexts refuse it)
+            const int vs = g.xc * g.yc * g.zc;
+            Raw v[NV];
+            if (b != BoundaryType::Constant) {
+                int s[3] = {c[0], c[1], c[2]};
+                s[Axis] = ghost_source(c[Axis], dims[Axis], g.ng, b);
+                const int src = (s[2] * g.yc + s[1]) * g.xc + s[0];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = Form::get(q, k * vs + src);
+                for (int k = 0; k < NV; ++k) v[k] = Form::get(q, k * vs + src);
+            }
+            ghost_modify<Form, ND, NV, Axis>(v, b, start, g.bconst[face]);
+            const int cell = (c[2] * g.yc + c[1]) * g.xc + c[0];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) Form::put(q, k * vs + cell, v[k]);
+            const bool kin = c[2] >= g.ng && c[2] < g.zc - g.ng;
+            const bool jin = c[1] >= g.ng && c[1] < g.yc - g.ng;
+            const bool final_value = Axis == 2 || (Axis == 1 && kin) || (Axis == 0 && kin && jin);
+            if (g.reduce_dt && final_value) a = Form::template signal_speed_nd<ND>(g.gamma, v);
+        }
     }
-    ghost_modify<Form, ND, NV, Axis>(v, b, start, g.bconst[face]);
-    const int cell = (c[2] * g.yc + c[1]) * g.xc + c[0];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) Form::put(q, k * vs + cell, v[k]);
+    if (g.reduce_dt) {
+        const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, a));
+        if ((threadIdx.x & 63) == 0 && m != 0) atomicMax(amax_bits, (unsigned long long)m);
+    }
 }
 
 }  // namespace mh

@@ -29,12 +29,14 @@ hipError_t launch_ghost_impl(int nd, const GhostGrid& g, typename Form::Planes i
 
 // 3-D ghost fill: the x, y and z passes in order on one buffer.
 template <class Form>
-hipError_t launch_ghost3d_impl(const Ghost3Grid& g, typename Form::Planes q, hipStream_t st) {
-    if (g.ng < 1 || g.xc < 3 * g.ng || g.yc < 3 * g.ng || g.zc < 3 * g.ng) return hipErrorInvalidValue;
+hipError_t launch_ghost3d_impl(const Ghost3Grid& g, typename Form::Planes q, unsigned long long* amax,
+                               hipStream_t st) {
+    if (g.ng < 1 || g.xc < 3 * g.ng || g.yc < 3 * g.ng || g.zc < 3 * g.ng || (g.reduce_dt && !amax))
+        return hipErrorInvalidValue;
     const long long nx = 2LL * g.ng * g.yc * g.zc, ny = 2LL * g.ng * g.xc * g.zc, nz = 2LL * g.ng * g.xc * g.yc;
-    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 0>), dim3(blocks_for(nx)), dim3(kBlock), 0, st, g, q);
-    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 1>), dim3(blocks_for(ny)), dim3(kBlock), 0, st, g, q);
-    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 2>), dim3(blocks_for(nz)), dim3(kBlock), 0, st, g, q);
+    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 0>), dim3(blocks_for(nx)), dim3(kBlock), 0, st, g, q, amax);
+    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 1>), dim3(blocks_for(ny)), dim3(kBlock), 0, st, g, q, amax);
+    hipLaunchKernelGGL((ghost_pass3d_kernel<Form, 2>), dim3(blocks_for(nz)), dim3(kBlock), 0, st, g, q, amax);
     return hipGetLastError();
 }
 
@@ -229,11 +231,11 @@ hipError_t launch_tab_eos_f32c(int nd, FormF32c::Planes in, FormF32c::Planes out
     return launch_tab_eos_impl<FormF32c>(nd, in, out, ncells, nvar, t, ytab, ltab, y_space, T_space, neg, st);
 }
 
-hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, hipStream_t st) {
-    return launch_ghost3d_impl<FormF64>(g, q, st);
+hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, unsigned long long* amax, hipStream_t st) {
+    return launch_ghost3d_impl<FormF64>(g, q, amax, st);
 }
-hipError_t launch_ghost3d_f32c(const Ghost3Grid& g, FormF32c::Planes q, hipStream_t st) {
-    return launch_ghost3d_impl<FormF32c>(g, q, st);
+hipError_t launch_ghost3d_f32c(const Ghost3Grid& g, FormF32c::Planes q, unsigned long long* amax, hipStream_t st) {
+    return launch_ghost3d_impl<FormF32c>(g, q, amax, st);
 }
 template <class Form>
 hipError_t launch_lte_impl(int nd, typename Form::Planes q, int ncells, const LteParams& p, double* y, double* T,

@@ -53,9 +53,10 @@ hipError_t launch_ghost_f64(int nd, const GhostGrid& g, FormF64::Planes in, Form
                             unsigned long long* amax, hipStream_t st);
 hipError_t launch_ghost_f32c(int nd, const GhostGrid& g, FormF32c::Planes in, FormF32c::Planes out,
                              unsigned long long* amax, hipStream_t st);
-// 3-D ghost fill of one buffer (x, y, z passes).
-hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, hipStream_t st);
-hipError_t launch_ghost3d_f32c(const Ghost3Grid& g, FormF32c::Planes q, hipStream_t st);
+// 3-D ghost fill of one buffer (x, y, z passes), folding the ghost cells
+// into the CFL max when g.reduce_dt is set.
+hipError_t launch_ghost3d_f64(const Ghost3Grid& g, FormF64::Planes q, unsigned long long* amax, hipStream_t st);
+hipError_t launch_ghost3d_f32c(const Ghost3Grid& g, FormF32c::Planes q, unsigned long long* amax, hipStream_t st);
 hipError_t launch_cfl_f64(int nd, FormF64::Planes q, int ncells, double gamma, unsigned long long* amax,
                           hipStream_t st);
 hipError_t launch_cfl_f32c(int nd, FormF32c::Planes q, int ncells, double gamma, unsigned long long* amax,
