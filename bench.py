@@ -126,11 +126,13 @@ def cpu_baseline(cfg, q0, seconds):
     sim.set_q(q0)
     sim.fill_bcs()
     t, p, steps = 0.0, 0.0, 0
+    oracle.profile(reset=True)
     t0 = time.perf_counter()
     while steps < 1 or time.perf_counter() - t0 < seconds:
         t, p, _ = sim.run(time=t, prev_output_time=p, max_steps=1)
         steps += 1
     el = time.perf_counter() - t0
+    breakdown = {k: round(v / steps, 2) for k, v in oracle.profile(reset=True).items()}
     cpu = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -143,7 +145,8 @@ def cpu_baseline(cfg, q0, seconds):
             "kind": "port",
             "sample": f"{steps} full SSPRK3 steps of the 2048^2 KH workload (oracle: the reference's unfused "
                       f"Q->W/Rx/Fx/Ry/Fy/combine/BC/dt sequence restated, g++ -O3 -fopenmp -ffp-contract=off), "
-                      f"{el:.1f} s on {threads} threads of {cpu}"}
+                      f"{el:.1f} s on {threads} threads of {cpu}",
+            "ms_per_step_by_kernel": breakdown}
 
 
 def copy_probe(mib: int) -> int:

@@ -73,6 +73,10 @@ def lib() -> C.CDLL:
             "orc_set_y_space": (None, [vp, vp]),
             "orc_update_eos": (None, [vp]),
             "orc_tab_sample": (None, [d, d, C.c_int32, d, d, C.c_int32, vp, vp, i64, vp, vp, vp, vp]),
+            "orc_profile_count": (C.c_int, []),
+            "orc_profile_name": (C.c_char_p, [C.c_int]),
+            "orc_profile_ms": (d, [C.c_int]),
+            "orc_profile_reset": (None, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -240,6 +244,15 @@ def riemann(solver, axis, nd, gamma, wl, wr):
     f = np.empty_like(a)
     lib().orc_riemann(SOLVER[solver], axis, nd, gamma, a.shape[0], _p(a), _p(b), _p(f))
     return f
+
+
+def profile(reset: bool = False) -> dict:
+    """Accumulated wall milliseconds per reference kernel name since the last reset."""
+    L = lib()
+    out = {L.orc_profile_name(k).decode(): L.orc_profile_ms(k) for k in range(L.orc_profile_count())}
+    if reset:
+        L.orc_profile_reset()
+    return {k: v for k, v in out.items() if v > 0.0}
 
 
 def tab_sample(t, log_eint_rho, log_rho):

@@ -15,11 +15,29 @@
 // with hand-worked examples) are marked "pinned by worked example".
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <limits>
 #include <vector>
+
+// Per-kernel wall-time accumulators named like the reference's kernels (the
+// dex_parallel_for names YAKL's auto-profiler reports), for the CPU
+// baseline's time breakdown.
+namespace prof {
+enum : int { QtoW, ReconX, FluxX, ReconY, FluxY, ReconZ, FluxZ, Integrate, FillBcs, Dt, Sources, QOld, UpdateEos, N };
+const char* const kNames[N] = {"Q -> W", "Reconstruct x", "Flux x", "Reconstruct y", "Flux y", "Reconstruct z",
+                               "Flux z", "integrate_flux (stage update)", "Fill BCs", "dt_reducer", "Source terms",
+                               "Q_old copy", "update_eos"};
+double ms[N] = {};
+struct Scope {
+    int k;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit Scope(int kk) : k(kk) {}
+    ~Scope() { ms[k] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+}  // namespace prof
 
 namespace {
 
@@ -329,6 +347,7 @@ inline size_t lin(const orc_sim& s, int v, int k, int j, int i) {
 
 // global_cons_to_prim_impl, Hydro.cpp:8-37 (over all cells incl. ghosts)
 void global_cons_to_prim(orc_sim& s) {
+    prof::Scope timer(prof::QtoW);
     const int nd = s.c.num_dim, n_hydro = 2 + nd, nv = s.nvar;
     const int64_t n = (int64_t)s.c.zc * s.c.yc * s.c.xc;
 #pragma omp parallel for schedule(static)
@@ -345,6 +364,7 @@ void global_cons_to_prim(orc_sim& s) {
 
 // compute_recon_impl, Hydro.cpp:55-102
 void compute_recon(orc_sim& s, int axis) {
+    prof::Scope timer(prof::ReconX + 2 * axis);
     const auto& c = s.c;
     int nx = c.xc - 2 * c.ng;
     int ny = std::max(c.yc - 2 * c.ng, 1);
@@ -377,6 +397,7 @@ void compute_recon(orc_sim& s, int axis) {
 
 // compute_flux_impl, Hydro.cpp:104-166
 void compute_flux(orc_sim& s, int axis) {
+    prof::Scope timer(prof::FluxX + 2 * axis);
     const auto& c = s.c;
     const int nd = c.num_dim, n_hydro = 2 + nd;
     int nx = c.xc - 2 * c.ng;
@@ -466,6 +487,7 @@ void tab_sample(const orc_tab& t, double log_eint_rho, double log_rho, double& y
 // and E += rho chi_H / m_H (y - y_prev) once y_prev has been set (the fields
 // start at -1, Eos.cpp:80).
 void update_eos_tab(orc_sim& s) {
+    prof::Scope timer(prof::UpdateEos);
     constexpr double h_mass = 1.6737830080950003e-27, chi_H = 2.178710282685096e-18;
     const int nd = s.c.num_dim;
     const size_t n = s.cells;
@@ -578,6 +600,7 @@ void fill_one_bc(orc_sim& s, int axis) {
 
 // fill_bcs_impl / fill_bcs, Boundaries.hpp:121-149 (no user_bc)
 void fill_bcs(orc_sim& s) {
+    prof::Scope timer(prof::FillBcs);
     fill_one_bc(s, 0);
     if (s.c.num_dim > 1) fill_one_bc(s, 1);
     if (s.c.num_dim > 2) fill_one_bc(s, 2);
@@ -587,6 +610,7 @@ void fill_bcs(orc_sim& s) {
 // of the current Q on every cell (S was zeroed before the stage,
 // SourceTerms.hpp:8-10).
 void compute_source_terms(orc_sim& s) {
+    prof::Scope timer(prof::Sources);
     const auto& c = s.c;
     if (!c.gravity) return;
     const int nd = c.num_dim;
@@ -612,6 +636,7 @@ void compute_source_terms(orc_sim& s) {
 // selected by (stepper, stage); the source term S (zero unless gravity is
 // registered) enters as S*dt for the hydro variables.
 void integrate_flux(orc_sim& s, int stepper, int stage, double dt) {
+    prof::Scope timer(prof::Integrate);
     const auto& c = s.c;
     const int nd = c.num_dim, n_hydro = 2 + nd;
     const int nx = c.xc - 2 * c.ng;
@@ -693,6 +718,7 @@ void orc_cons_to_prim_all(orc_sim* s) { global_cons_to_prim(*s); }
 // reduction result into dt_max (the 1e5 seed only survives an empty range).
 double orc_compute_dt(const orc_sim* s, double max_cfl, double time, double max_time,
                       double prev_output_time, double delta) {
+    prof::Scope timer(prof::Dt);
     const auto& c = s->c;
     const int nd = c.num_dim;
     const int64_t n = (int64_t)c.zc * c.yc * c.xc;
@@ -727,9 +753,15 @@ double orc_compute_dt(const orc_sim* s, double max_cfl, double time, double max_
 // fill_bcs. Q_old is a copy of Q at the start of the step.
 void orc_time_step(orc_sim* s, double dt) {
     const int ns = num_stages(s->c.stepper);
-    s->Q_old = s->Q;
+    {
+        prof::Scope timer(prof::QOld);
+        s->Q_old = s->Q;
+    }
     for (int st = 0; st < ns; ++st) {
-        std::fill(s->S.begin(), s->S.end(), 0.0);  // zero_source_terms, every stage
+        {
+            prof::Scope timer(prof::Sources);
+            std::fill(s->S.begin(), s->S.end(), 0.0);  // zero_source_terms, every stage
+        }
         if (st > 0) fill_bcs(*s);
         compute_hydro_fluxes(*s);
         compute_source_terms(*s);
@@ -864,6 +896,14 @@ void orc_tab_sample(double ex_start, double ex_step, int32_t ex_num, double r_st
     t.y.assign(y_table, y_table + m);
     t.log_T.assign(log_T_table, log_T_table + m);
     for (int64_t i = 0; i < n; ++i) tab_sample(t, log_eint_rho[i], log_rho[i], y[i], log_T[i]);
+}
+
+// ---- per-kernel time breakdown ----------------------------------------------
+int orc_profile_count() { return prof::N; }
+const char* orc_profile_name(int k) { return (k >= 0 && k < prof::N) ? prof::kNames[k] : ""; }
+double orc_profile_ms(int k) { return (k >= 0 && k < prof::N) ? prof::ms[k] : 0.0; }
+void orc_profile_reset() {
+    for (double& v : prof::ms) v = 0.0;
 }
 
 // ---- single-functor entry points for the unit tests ------------------------
