@@ -424,8 +424,8 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs32 / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("void mh::stage2d_kernel<mh::FormF32c, (mh::Reconstruction)3, "
-                                                 "(mh::SlopeLimiter)1, (mh::RiemannSolver)2>"),
-                     "kernel": "stage2d_kernel<FormF32c, Weno5Z, HLLC>",
+                                                 "(mh::SlopeLimiter)1, (mh::RiemannSolver)2, false>"),
+                     "kernel": "stage2d_kernel<FormF32c, Weno5Z, MC, HLLC, false>",
                      "avg_launch_ms": round(avg32, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "autotune": {"fp32c": tune32, "fp64": tune64},
         "kernel_split_ms_per_step": {
