@@ -180,3 +180,24 @@ def test_sedov_3d_through_host_driver(orc, tmp_path):
         times, Q = f.variables["time"][:].copy(), f.variables["Q"][:].copy()
     assert times[-1] == t_o
     np.testing.assert_array_equal(Q[-1], want)
+
+
+@pytest.mark.parametrize("form,tol", [(M.FP64, 1e-13), (M.FP32C, 1e-6)])
+def test_conservation_256_cubed(form, tol):
+    """At the 3-D bench size (256^3, periodic) every interior sum of rho, m and E is conserved: the
+    z sweep's face fluxes and the march's x/y faces telescope, so only rounding remains."""
+    print(f"case: 3-D conservation 256^3 form {form}", flush=True)
+    cfg = Config(num_dim=3, nx=256, ny=256, nz=256, ng=3, x_dim=1.0, bc=("periodic",) * 6, reconstruction="weno5z",
+                 riemann_solver="hllc", time_stepper="ssprk3", max_cfl=0.3)
+    q0 = problems.blast_3d(cfg, seed=8, p_ratio=3.0)
+    g = cfg.ng
+    with M.HydroContext(cfg, precision_form=form) as ctx:
+        ctx.upload(q0)
+        ctx.fill_bcs()
+        s0 = ctx.download()[:, g:-g, g:-g, g:-g].reshape(5, -1).sum(axis=1)
+        ctx.run(5)
+        q = ctx.download()
+    assert np.isfinite(q).all()
+    s1 = q[:, g:-g, g:-g, g:-g].reshape(5, -1).sum(axis=1)
+    scale = np.abs(q0[:, g:-g, g:-g, g:-g]).reshape(5, -1).sum(axis=1)
+    np.testing.assert_allclose(s1, s0, rtol=0, atol=tol * scale.max())
