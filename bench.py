@@ -295,14 +295,14 @@ def bench_sedov(a):
 
 
 def bench_blast3d(a):
-    """3-D blast (WENO5-Z + HLLC + SSPRK3, periodic, CFL 0.4) on an n^3 grid: FP64 and FP32C
+    """3-D blast (WENO5-Z + HLLC + SSPRK3, periodic, CFL 0.3) on an n^3 grid: FP64 and FP32C
     throughput of graph-replayed steps, FP32C-vs-FP64 rel L1 after the timed steps, and the
     algorithmic bandwidth (5 variables x 8 B x (2 x 3 + 2) = 320 B per cell-step)."""
     import mosscap_amd as M
     from mosscap_amd import Config, problems
     n = a.n3
     cfg = Config(num_dim=3, nx=n, ny=n, nz=n, ng=3, x_dim=1.0, bc=("periodic",) * 6, reconstruction="weno5z",
-                 riemann_solver="hllc", time_stepper="ssprk3", max_cfl=0.4, output_delta=1e30, max_time=1e30)
+                 riemann_solver="hllc", time_stepper="ssprk3", max_cfl=0.3, output_delta=1e30, max_time=1e30)
     q0 = problems.blast_3d(cfg, seed=3)
     res, states = {}, {}
     for form in (M.FP64, M.FP32C):
@@ -315,7 +315,7 @@ def bench_blast3d(a):
             states[form] = ctx.download()
     cells = n ** 3
     out = {"metric": "3-D blast Mcell-updates/s", "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
-           "config": {"workload": f"3-D blast {n}^3 periodic, ng=3, WENO5-Z + HLLC + SSPRK3, CFL 0.4, "
+           "config": {"workload": f"3-D blast {n}^3 periodic, ng=3, WENO5-Z + HLLC + SSPRK3, CFL 0.3, "
                                   "hipGraph-replayed steps (z sweep + per-plane x-y march)"}}
     for form, name in ((M.FP64, "fp64"), (M.FP32C, "fp32c")):
         el, tm, tune = res[form]
@@ -334,8 +334,8 @@ def bench_blast3d(a):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=2048, help="grid size (the workload is 2048)")
     ap.add_argument("--parity-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
