@@ -367,8 +367,8 @@ def main(argv=None):
     if a.copy_probe_mb:
         return copy_probe(a.copy_probe_mb)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; a multi-GPU figure is not part of it "
-              "(halo-exchange sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU; multi-GPU runs are not supported (a multi-GPU run would shard "
+              "the grid in slabs with a halo exchange per stage)", file=sys.stderr)
         return 2
 
     import mosscap_amd as M

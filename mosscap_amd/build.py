@@ -3,7 +3,7 @@
     python -m mosscap_amd.build            # both
     python -m mosscap_amd.build --lib      # HIP library only
 
-Outputs (git-ignored, shipped to the GPU box with the tree):
+Outputs (git-ignored; built in-tree so they travel with the source tree):
     mosscap_amd/libmosscap_hip.so   the product: kernels + C ABI (include/mosscap_hip.h)
     mosscap_amd/libmosscap_host.so  host layer: YAML setup, problems, output, driver (include/mosscap_host.h)
     mosscap_amd/mosscap_hip         command-line driver (the reference's main.cpp)

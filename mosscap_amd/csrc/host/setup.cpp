@@ -42,7 +42,7 @@ void setup_grid(Simulation& sim, const Config& config) {
     sim.state.loc.z = get_or<double>(config, "grid.z_start", 0.0);
 
     if (get_or<bool>(config, "dex.enable", false))
-        throw std::runtime_error("dex.enable: the radiative-transfer coupling is not part of this build");
+        throw std::runtime_error("dex.enable: the radiative-transfer coupling is not supported by this library");
     sim.state.num_tracers = get_or<int>(config, "simulation.n_extra_fields", 0);
     if (sim.state.num_tracers < 0) throw std::runtime_error("simulation.n_extra_fields cannot be negative");
     sim.state.nvar = num_hydro_vars(sim.num_dim) + sim.state.num_tracers;
@@ -165,7 +165,7 @@ void setup_eos(Simulation& sim, const Config& config) {
         sim.eos.tabulated = true;
         sim.eos.is_constant = false;
     } else {
-        throw std::runtime_error("eos.type " + type + ": the DexRT equation of state is not part of this build");
+        throw std::runtime_error("eos.type " + type + ": the DexRT equation of state is not supported by this library");
     }
 }
 

@@ -171,7 +171,7 @@ def test_constant_bc_mapping_1d(H):
     (MIN + "boundary:\n  xs: outflow\n", r'for "outflow"'),
     (MIN + "boundary:\n  xs: constant\n  xs_const: xxx\n", r"Provide constant boundary values for xs_const"),
     (MIN + "eos:\n  type: tabulatedlteh\n", r"eos.table_path mosscap_lte_h_tables.nc: cannot open the file"),
-    (MIN + "eos:\n  type: dexpressure\n", r"DexRT equation of state is not part"),
+    (MIN + "eos:\n  type: dexpressure\n", r"DexRT equation of state is not supported"),
     (MIN + "output:\n  variables:\n    fluxes: true\n", r"fluxes / source"),
     ("simulation:\n  num_dim: 2\n", r"No method for circular_explosion found."),
     (MIN + "grid:\n  dx: -1.0\n", r"grid.dx cannot be negative."),
