@@ -30,6 +30,10 @@ int main(int argc, char** argv) {
     }
     try {
         const mh::host::Config config = mh::host::load_yaml_file(config_path);
+        // The reference sizes YAKL's memory pool from this key (main.cpp:44-47); the
+        // library allocates its fixed buffers once, so the value is read (with the
+        // same conversion errors) and not used.
+        (void)mh::host::get_or<double>(config, "system.mem_pool_gb", 2.0);
         mh::host::Simulation sim = mh::host::setup_sim_host(config);
         sim.out_cfg.directory = out_dir;
         mh::host::attach_device(sim, true);
