@@ -6,6 +6,7 @@
 #include <cstring>
 #include <string>
 
+#include "config.hpp"
 #include "simulation.hpp"
 
 int main(int argc, char** argv) {
