@@ -404,3 +404,11 @@ def test_lte_table_reader_formats(H, tmp_path):
         v[()] = 1.0
     with pytest.raises(H.HostError, match="no variable log_eint_rho_step"):
         H.setup_summary(base.format(p3))
+
+
+def test_driver_executable_help():
+    """The command-line driver links against both libraries and answers --help without a GPU."""
+    import subprocess
+    exe = Path(__file__).resolve().parent.parent / "mosscap_amd" / "mosscap_hip"
+    out = subprocess.run([str(exe), "--help"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "usage: mosscap_hip" in out.stdout
