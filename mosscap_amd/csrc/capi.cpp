@@ -149,9 +149,9 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
         if (c->fp32c()) e = hipMalloc(&c->dz32, n * sizeof(float));
         else e = hipMalloc(&c->dz64, n * sizeof(double));
     }
-    if (e == hipSuccess) e = hipMalloc(&c->clock, sizeof(Clock));
+    if (e == hipSuccess) e = hipMalloc(&c->clock, kClockBytes);
     if (e == hipSuccess) e = hipMalloc(&c->floor_hits, sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMemsetAsync(c->clock, 0, sizeof(Clock), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->clock, 0, kClockBytes, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->floor_hits, 0, sizeof(unsigned int), c->stream);
     // defined contents everywhere (ghosts of the rotating buffers included)
     for (int k = 0; k < 3 && e == hipSuccess; ++k) {
@@ -302,9 +302,9 @@ static hipError_t write_clock(mh_ctx* c, double max_cfl, double time, double max
     h.delta = delta;
     h.max_cfl = max_cfl;
     h.dx = c->dx;
-    h.amax_bits = 0ull;
     h.step = c->steps;
-    return hipMemcpyAsync(c->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream);
+    const hipError_t e = hipMemcpyAsync(c->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream);
+    return e == hipSuccess ? launch_clock_reset_max(c->clock, c->stream) : e;
 }
 
 // The tabulated EOS's abort condition (TabulatedLteH.hpp:140-142) after a

@@ -171,7 +171,7 @@ struct mh_ctx {
     // Ghost fill of `out` after a stage from `in`. In 3-D the three axis
     // passes run on `out` alone.
     hipError_t ghost(int in, int out, int reduce) {
-        unsigned long long* amax = &clock->amax_bits;
+        unsigned long long* amax = amax_slots(clock);
         hipError_t e = timed(1, [&] {
             if (nd == 3)
                 return fp32c() ? launch_ghost3d_f32c(ghost3_grid(reduce), p32(out), amax, stream)
@@ -194,7 +194,7 @@ struct mh_ctx {
     }
 
     hipError_t cfl_all(int k) {
-        unsigned long long* amax = &clock->amax_bits;
+        unsigned long long* amax = amax_slots(clock);
         if (fp32c()) return launch_cfl_f32c(nd, p32(k), (int)ncells, gamma, amax, stream);
         return launch_cfl_f64(nd, p64(k), (int)ncells, gamma, amax, stream);
     }
@@ -238,7 +238,7 @@ struct mh_ctx {
         const TracerGrid tg{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, ntr, kind};
         if (fp32c()) {
             FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), &clock->dt, dx, gamma, gamma - 1.0,
-                             &clock->amax_bits, floor_hits, opt.rho_floor, opt.p_floor, frx32, fry32,
+                             amax_slots(clock), floor_hits, opt.rho_floor, opt.p_floor, frx32, fry32,
                              gravity, grav[0], grav[1]};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
@@ -251,7 +251,7 @@ struct mh_ctx {
                 e = launch_stage1d_f32c(sch, kind, reduce, sz.xc, sz.ng, p, stream);
             }
         } else {
-            FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, &clock->amax_bits, nullptr,
+            FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, amax_slots(clock), nullptr,
                             frx64, fry64, gravity, grav[0], grav[1]};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
@@ -270,7 +270,7 @@ struct mh_ctx {
     // FP64 stage between arbitrary device buffers (adjoint recomputation).
     hipError_t stage_ptr(const double* in, const double* qn_p, double* out, int kind) {
         FormF64::Ptrs p{FormF64::Planes{(double*)in}, FormF64::Planes{(double*)qn_p}, FormF64::Planes{out}, &clock->dt,
-                        dx, gamma, &clock->amax_bits, nullptr, nullptr, nullptr, gravity, grav[0], grav[1]};
+                        dx, gamma, amax_slots(clock), nullptr, nullptr, nullptr, gravity, grav[0], grav[1]};
         StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, 0};
         hipError_t e = nd == 2 ? launch_stage2d_f64(sch, g, p, stream)
                                : launch_stage1d_f64(sch, kind, 0, sz.xc, sz.ng, p, stream);

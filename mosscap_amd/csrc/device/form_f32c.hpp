@@ -214,12 +214,27 @@ struct FormF32c {
         }
     }
 
+    // sqrt(x) to ~1e-14 relative: the FP32 reciprocal square root refined
+    // by one FP64 Newton step, s + (x - s^2) y / 2 with s = x y. Zero gives
+    // zero; values outside [1e-30, 1e30] (FP32's range with margin), negative
+    // or NaN ones take the correctly rounded FP64 square root.
+    __device__ static inline double sqrt_fast(double x) {
+        const bool fast = x > 1e-30 && x < 1e30;
+        if (!fast && x != 0.0) return __builtin_sqrt(x);
+        const double xs = fast ? x : 1.0;
+        const double y = (double)__builtin_amdgcn_rsqf((float)xs);
+        const double s = xs * y;
+        const double r = __builtin_fma(0.5 * y, __builtin_fma(-s, s, xs), s);
+        return fast ? r : 0.0;
+    }
+
     // CFL signal speed in FP64; a state whose pressure went negative
     // contributes its sound speed at p = 0 (the floors keep the scheme going).
     template <int ND, int NV>
     __device__ static inline double signal_speed_nd(double gamma, const float2 (&q)[NV]) {
         // FP64 throughout, with 1/rho from the FP32 reciprocal and one Newton
-        // step (relative error ~1e-14) in place of three IEEE divisions.
+        // step (relative error ~1e-14) in place of three IEEE divisions, and
+        // the square roots from sqrt_fast: |v| = sqrt(|m|^2) / rho.
         double rho = df::to_double(q[0]);
         rho = rho > 1e-300 ? rho : 1e-300;
         const double r0 = (double)__builtin_amdgcn_rcpf((float)rho);
@@ -230,10 +245,9 @@ struct FormF32c {
             const double m = df::to_double(q[1 + a]);
             m2 += m * m;
         }
-        const double v2 = m2 * r * r;
         double pres = (gamma - 1.0) * (df::to_double(q[1 + ND]) - 0.5 * m2 * r);
         pres = pres > 0.0 ? pres : 0.0;
-        return __builtin_sqrt(gamma * pres * r) + __builtin_sqrt(v2);
+        return sqrt_fast(gamma * pres * r) + sqrt_fast(m2) * r;
     }
 };
 

@@ -36,9 +36,31 @@ struct Clock {
     double delta;
     double max_cfl;
     double dx;
-    unsigned long long amax_bits;  // running max of c_s + |v| (bit pattern)
     long long step;
 };
+
+// The running CFL max (bit pattern of the largest c_s + |v|; non-negative
+// doubles order like their bit patterns) lives in kAmaxSlots words that follow
+// the Clock in its allocation, each on a 256-byte line of its own. A wave
+// folds its maximum into the slot its block index selects and the step's
+// finalize reduces the slots. Device-scope atomics on one address are carried
+// out one after another (about 12 ns apart on MI355X), so the ~4000 waves of a
+// 2048^2 stage folding into a single word would queue for tens of
+// microseconds while the launch drains.
+constexpr int kAmaxSlots = 64;         // one per lane of the finalize wave
+constexpr int kAmaxStride = 32;        // 64-bit words per slot (256 B)
+constexpr size_t kAmaxOffset = 256;    // byte offset of slot 0 from the Clock
+constexpr size_t kClockBytes = kAmaxOffset + sizeof(unsigned long long) * kAmaxSlots * kAmaxStride;
+static_assert(sizeof(Clock) <= kAmaxOffset, "the Clock overlaps its CFL slots");
+
+__host__ __device__ inline unsigned long long* amax_slots(Clock* c) {
+    return reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(c) + kAmaxOffset);
+}
+
+// Fold a wave's maximum into the slot of `block` (lane 0 of the wave calls).
+__device__ inline void fold_amax(unsigned long long* slots, uint32_t block, uint64_t m) {
+    atomicMax(slots + kAmaxStride * (block & (uint32_t)(kAmaxSlots - 1)), (unsigned long long)m);
+}
 
 // Source index along an axis of extent `dims` for ghost coordinate `g`
 // (fill_one_bc_impl, Boundaries.hpp:27-57).
@@ -148,12 +170,16 @@ __global__ __launch_bounds__(256) void ghost_fill_kernel(GhostGrid g, typename F
         double a = 0.0;
         if (owner) a = Form::template signal_speed_nd<ND>(g.gamma, q);
         const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, a));
-        if ((threadIdx.x & 63) == 0 && m != 0) atomicMax(amax_bits, (unsigned long long)m);
+        if ((threadIdx.x & 63) == 0 && m != 0) {
+            // 1-D fills also serve single-word maxima (one per batched problem)
+            if constexpr (ND > 1) fold_amax(amax_bits, blockIdx.x, m);
+            else atomicMax(amax_bits, (unsigned long long)m);
+        }
     }
 }
 
 // CFL reduction over every cell, ghosts included (compute_dt_impl,
-// Hydro.cpp:208-229): max of c_s + |v|, folded into clock->amax_bits.
+// Hydro.cpp:208-229): max of c_s + |v|, folded into the clock's CFL slots.
 template <class Form, int ND>
 __global__ __launch_bounds__(256) void cfl_reduce_kernel(typename Form::Planes q_in, int ncells, double gamma,
                                                          unsigned long long* amax_bits) {
@@ -168,7 +194,7 @@ __global__ __launch_bounds__(256) void cfl_reduce_kernel(typename Form::Planes q
         amax = a > amax ? a : amax;
     }
     const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
-    if ((threadIdx.x & 63) == 0 && m != 0) atomicMax(amax_bits, (unsigned long long)m);
+    if ((threadIdx.x & 63) == 0 && m != 0) fold_amax(amax_bits, blockIdx.x, m);
 }
 
 }  // namespace mh

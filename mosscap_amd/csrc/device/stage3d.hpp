@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void ghost_pass3d_kernel(Ghost3Grid g, typenam
     }
     if (g.reduce_dt) {
         const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, a));
-        if ((threadIdx.x & 63) == 0 && m != 0) atomicMax(amax_bits, (unsigned long long)m);
+        if ((threadIdx.x & 63) == 0 && m != 0) fold_amax(amax_bits, blockIdx.x, m);
     }
 }
 

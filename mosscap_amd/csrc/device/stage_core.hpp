@@ -30,6 +30,7 @@
 #pragma once
 
 #include "functors.hpp"
+#include "ghost_dt.hpp"
 #include "wave.hpp"
 
 namespace mh {
@@ -264,7 +265,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
     if (g.reduce_dt) {
         const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
-        if (lane == 0) atomicMax((unsigned long long*)p.amax_bits, (unsigned long long)m);
+        if (lane == 0) fold_amax((unsigned long long*)p.amax_bits, blockIdx.x, m);
     }
     if (p.floor_hits != nullptr) {
         const unsigned long long any = __ballot(floor_hits != 0u);

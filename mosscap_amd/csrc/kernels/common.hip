@@ -166,9 +166,15 @@ hipError_t probe_riemann_t(int solver, int axis, int nd, double gamma, int n, co
 // (Hydro.cpp:230-241). With `advance`, also runs main.cpp:54-71's bookkeeping:
 // time += dt and, once an output time is reached, prev_output_time = time.
 // Resets the running max for the next step.
+// One wave: lane l reduces and clears slot l.
 __global__ void clock_finalize_kernel(Clock* c, int advance) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double amax = __builtin_bit_cast(double, c->amax_bits);
+    static_assert(kAmaxSlots == 64, "one slot per lane");
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+    unsigned long long* slot = amax_slots(c) + kAmaxStride * threadIdx.x;
+    const uint64_t bits = wave_max_u64(*slot);
+    *slot = 0ull;
+    if (threadIdx.x != 0) return;
+    const double amax = __builtin_bit_cast(double, bits);
     const double dt_min = c->dx / amax;  // == min over cells of dx / a
     double dt = dt_min * c->max_cfl;
     const double t = c->time;
@@ -180,7 +186,6 @@ __global__ void clock_finalize_kernel(Clock* c, int advance) {
         }
     }
     c->dt = dt;
-    c->amax_bits = 0ull;
     if (advance) {
         c->time = t + dt;
         c->step += 1;
@@ -193,7 +198,7 @@ __global__ void clock_set_dt_kernel(Clock* c, double dt) {
 }
 
 __global__ void clock_reset_max_kernel(Clock* c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) c->amax_bits = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < kAmaxSlots) amax_slots(c)[kAmaxStride * threadIdx.x] = 0ull;
 }
 
 }  // namespace
