@@ -91,11 +91,19 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int plane = g.xc * g.yc * (ND > 2 ? g.zc : 1);
     const int pk = ND > 2 ? (g.ng + (int)(bb / per_plane)) * g.xc * g.yc : 0;
 
-    // Halo lanes: lanes [0, ORD) fill entries [0, ORD) (columns left of the
-    // strip), lanes [ORD, 2*ORD) fill entries [64+ORD, 64+2*ORD).
-    const bool halo_lane = lane < 2 * ORD;
+    // Halo columns: ring entries [0, ORD) (columns left of the strip) and
+    // [64+ORD, 64+2*ORD) (right of it). They are converted to primitives in
+    // batches of HB rows, one lane per (row, column) pair, into `hbuf`, and
+    // lanes [0, 2*ORD) copy a committed row's entries into the ring: one
+    // cons->prim pass per HB rows instead of a divergent pass per row.
+    constexpr int NH = 2 * ORD;
+    constexpr int HB = NH > 0 ? kWave / NH : 1;  // 16 rows for ORD 2, 32 for ORD 1
+    static_assert((HB & (HB - 1)) == 0, "batch rows must be a power of two");
+    __shared__ T hbuf[HB][NV][NH > 0 ? NH : 1];
+    const bool halo_lane = lane < NH;
     const int h_entry = lane < ORD ? lane : kWave + lane;
-    const int h_col = min(i0 - 1 - ORD + h_entry, g.xc - 1);
+    const int r_first = j0 - 1 - ORD;  // first and last rows the march commits
+    const int r_last = j1 + ORD;
 
     // The plain instance addresses Q through buffer resources: scalar offsets
     // for the wave-uniform row/plane/variable part, fixed per-lane vector
@@ -120,27 +128,52 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
     auto commit_row = [&](int r, const Raw (&q)[NV]) {
         const int slot = r & (RING - 1);
+        // halo entries of row r: read from the batch first, so that the LDS
+        // latency hides behind this row's cons->prim
+        T hv[NV];
+        if constexpr (NH > 0) {
+            const int hr = (r - r_first) & (HB - 1);
+            if (hr == 0) {  // wave-uniform: convert the halo of rows r .. r+HB-1
+                wave_lds_sync();  // the previous batch's reads are done
+                const int hb_idx = lane % NH;  // this lane's halo column
+                const int hb_row = lane / NH;  // and row offset in the batch
+                const int hb_col = min(i0 - 1 - ORD + (hb_idx < ORD ? hb_idx : kWave + hb_idx), g.xc - 1);
+                const int rr = r + hb_row;
+                if (rr <= r_last) {
+                    Raw qh[NV];
+                    // the lane's row goes in the vector offset (the scalar one is wave-uniform)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qh[v] = load_qs(rr * xc + hb_col, v * plane + pk);
+                    T wh[NV];
+                    Form::template to_prim_nd<ND>(p, qh, wh);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) hbuf[hb_row][v][hb_idx] = wh[v];
+                }
+                wave_lds_sync();
+            }
+            if (halo_lane) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) hv[v] = hbuf[hr][v][lane];
+            }
+        }
         T w[NV];
         const bool hit = Form::template to_prim_nd<ND>(p, q, w);
         // each interior cell is committed by exactly one (strip, span) owner
         floor_hits += (hit && out_lane && r >= j0 && r < j1) ? 1u : 0u;
 #pragma unroll
         for (int v = 0; v < NV; ++v) ring[slot][v][lane + ORD] = w[v];
+        if constexpr (NH > 0) {
+            if (halo_lane) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ring[slot][v][h_entry] = hv[v];
+            }
+        }
 #pragma unroll
         for (int k = 0; k + 1 < WIN; ++k)
 #pragma unroll
             for (int v = 0; v < NV; ++v) win[k][v] = win[k + 1][v];
 #pragma unroll
         for (int v = 0; v < NV; ++v) win[WIN - 1][v] = w[v];
-        if (ORD > 0 && halo_lane) {
-            Raw qh[NV];  // halo columns: loaded at commit, not prefetched (registers)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qh[v] = load_qs(h_col, v * plane + pk + r * xc);
-            T wh[NV];
-            Form::template to_prim_nd<ND>(p, qh, wh);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) ring[slot][v][h_entry] = wh[v];
-        }
     };
     auto fetch_row = [&](int r, Raw (&q)[NV]) {
         const int base = pk + r * xc;
