@@ -8,9 +8,9 @@
 // * cons->prim recovers the pressure p = (gamma-1)(E - |m|^2/(2 rho)) in FP64
 //   from the full hi+lo values: at high Mach number E and the kinetic energy
 //   nearly cancel, which FP32 cannot resolve.
-// * The stage combine is done in double-float arithmetic (TwoSum, and TwoProd
-//   through FMA) with double-float stage coefficients, so the state
-//   accumulates the small per-stage increments without FP32 rounding drift.
+// * The stage combine adds each stage's increment over Q^n, formed in FP32,
+//   to the hi+lo Q^n with TwoSum, so the state accumulates the small
+//   per-stage increments without FP32 rounding drift.
 // * The CFL signal speed is evaluated in FP64 from hi+lo.
 // * Optional positivity floors on rho and p of the reconstruction input; each
 //   interior cell and stage that needed one is counted once.
@@ -38,27 +38,12 @@ __device__ inline float2 fast_two_sum(float a, float b) {  // requires |a| >= |b
     const float e = b - (s - a);
     return make_float2(s, e);
 }
-__device__ inline float2 add(float2 a, float2 b) {
-#pragma clang fp contract(off)
-    const float2 s = two_sum(a.x, b.x);
-    const float e = s.y + (a.y + b.y);
-    return fast_two_sum(s.x, e);
-}
 __device__ inline float2 add_f(float2 a, float b) {
 #pragma clang fp contract(off)
     const float2 s = two_sum(a.x, b);
     const float e = s.y + a.y;
     return fast_two_sum(s.x, e);
 }
-// a * c for a double-float constant c = (c.x, c.y)
-__device__ inline float2 mul_c(float2 a, float2 c) {
-#pragma clang fp contract(off)
-    const float p = a.x * c.x;
-    const float e0 = __builtin_fmaf(a.x, c.x, -p);
-    const float e = e0 + (a.x * c.y + a.y * c.x);
-    return fast_two_sum(p, e);
-}
-__device__ inline float2 scale_pow2(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ inline double to_double(float2 a) { return (double)a.x + (double)a.y; }
 __device__ inline float2 from_double(double d) {
     const float hi = (float)d;
@@ -182,25 +167,29 @@ struct FormF32c {
         src[1 + ND] = (float)(energy_update * d);
     }
 
-    // The stage lambdas of TimeStepping.cpp in double-float arithmetic, with
-    // the source joined to the flux increment (dq + S*dt) in FP32.
-    // Coefficients 1/3, 2/3 and 1/6 are carried as float pairs (hi + lo),
-    // exact to ~1e-16.
+    // The stage lambdas of TimeStepping.cpp, a_qn Q^n + a_qs Q + a_l dq,
+    // rearranged around Q^n where they read it (a_qn + a_qs = 1):
+    //   Q^n + a_qs ((Q - Q^n) + (a_l / a_qs) dq).
+    // The increment over Q^n is formed in FP32 (Q - Q^n is exact in the hi
+    // words while Q stays within a factor 2 of Q^n, and FP32's relative error
+    // on the increment is what the FP32 fluxes carry anyway) and added to
+    // Q^n with TwoSum, so the hi+lo state accumulates the small per-stage
+    // increments without FP32 rounding drift. The source joins the flux
+    // increment (dq + S*dt) in FP32.
     __device__ static inline float2 combine(int kind, float2 Q, float2 Qn, float dq_flux, float src) {
+#pragma clang fp contract(off)
         const float dq = dq_flux + src;
-        const float2 third = make_float2(0x1.555556p-2f, -0x1.555556p-27f);
-        const float2 two_thirds = make_float2(0x1.555556p-1f, -0x1.555556p-26f);
-        const float2 sixth = make_float2(0x1.555556p-3f, -0x1.555556p-28f);
+        const auto around_qn = [&](float a_qs, float b) {
+            const float d = (Q.x - Qn.x) + (Q.y - Qn.y);
+            return df::add_f(Qn, a_qs * (d + b * dq));
+        };
         switch (kind) {
-            case 0: return df::add_f(Q, dq);
-            case 1: return df::scale_pow2(df::add_f(df::add(Qn, Q), dq), 0.5f);
-            case 2: return df::add(df::mul_c(Qn, make_float2(0.75f, 0.0f)), df::scale_pow2(df::add_f(Q, dq), 0.25f));
-            case 3: return df::add(df::mul_c(Qn, third), df::mul_c(df::add_f(Q, dq), two_thirds));
-            case 4: return df::add_f(Q, 0.5f * dq);
-            default: {
-                const float2 a = df::add(df::mul_c(Qn, two_thirds), df::mul_c(Q, third));
-                return df::add(a, df::mul_c(make_float2(dq, 0.0f), sixth));
-            }
+            case 0: return df::add_f(Q, dq);                    // Q + dq
+            case 1: return around_qn(0.5f, 1.0f);               // (Q^n + Q + dq) / 2
+            case 2: return around_qn(0.25f, 1.0f);              // 3/4 Q^n + 1/4 (Q + dq)
+            case 3: return around_qn(2.0f / 3.0f, 1.0f);        // 1/3 Q^n + 2/3 (Q + dq)
+            case 4: return df::add_f(Q, 0.5f * dq);             // Q + dq / 2
+            default: return around_qn(1.0f / 3.0f, 0.5f);       // 2/3 Q^n + 1/3 Q + 1/6 dq
         }
     }
 
