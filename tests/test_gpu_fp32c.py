@@ -58,6 +58,11 @@ def test_kh_512_200_steps_rel_l1():
     print("KH 512^2 200 steps FP32C vs FP64 rel L1:", r)
     assert r[0] <= 1e-5, r
     assert r[1] <= 1e-5 and r[3] <= 1e-5 and r[2] <= 5e-4, r
+    # Regression guard for the compensated state update (the stage's FP32
+    # increment over Q^n added with TwoSum): measured 5.5e-8 (rho) and 5.9e-8
+    # (E); a combine that lost the low words would sit near FP32's 1e-7 per
+    # stage and grow with the step count.
+    assert r[0] <= 3e-7 and r[3] <= 3e-7, r
 
 
 @pytest.mark.parametrize("recon,lim,solver,stepper", [("muscl", "minmod", "hllc", "rk2"),
