@@ -1,7 +1,7 @@
 #!/usr/bin/env python
 """Headline benchmark: Mcell-updates/s of the fused HIP hydro step on one MI355X.
 
-Workload (BASELINE config "2D Kelvin-Helmholtz, plain method at scale"):
+Workload (2-D Kelvin-Helmholtz, plain method at scale):
 2048x2048 periodic on [0,1]^2, 3 ghost cells, gamma 1.4, WENO5-Z + HLLC +
 SSPRK3, CFL 0.4, seeded shear-band initial state. A step is one full SSPRK3
 step: 3 fused stages, 3 ghost fills, the CFL dt (fused into the last stage)

@@ -163,7 +163,7 @@ def blast_3d(cfg: Config, seed: int = 7, amp: float = 0.05, p_ratio: float = 10.
     return prim_to_cons_3d(cfg, rho, vx, vy, vz, p)
 
 
-# BASELINE configurations, with the reference's key names.
+# The benchmark workloads (Sod 256^2, KH 2048^2, Sedov 4096^2), with the reference's key names.
 def sod_config() -> Config:
     return Config(num_dim=2, nx=256, ny=256, ng=2, x_dim=1.0, bc=("zerograd",) * 4, reconstruction="muscl",
                   slope_limiter="minmod", riemann_solver="hllc", time_stepper="rk2", gamma=1.4, max_cfl=0.4,

@@ -1,7 +1,7 @@
 """The compensated FP32 precision form against the FP64 path (= oracle).
 
 Tolerances (rel L1 per conserved variable, interior cells): density <= 1e-5
-on the KH workload (the BASELINE target). The y momentum of the KH band is
+on the KH workload (the precision form's target). The y momentum of the KH band is
 only the 1% seed perturbation, so its own L1 norm is ~100x smaller than the
 x momentum's and its relative error is held to 5e-4; variables that are
 identically zero in the reference (m_y of the Sod tube) must stay exactly 0.

@@ -188,7 +188,7 @@ def test_step_parity_1d(orc, recon, solver, stepper):
 
 
 def test_sod_256_to_t_end(orc):
-    """2-D Sod 256^2, MUSCL-minmod + HLLC + RK2, CFL 0.4, t_end 0.2."""
+    """2-D Sod 256^2, MUSCL-minmod + HLLC + RK2, CFL 0.4, to t_end 0.2."""
     cfg = problems.sod_config()
     q0 = problems.shock_tube_2d(cfg)
     sim = orc.OracleSim(cfg)
@@ -262,7 +262,7 @@ def test_stagewise_parity(orc):
 
 
 def test_kh_2048_full_size_one_step(orc):
-    """BASELINE config 2 at full size: one SSPRK3 step bitwise against the oracle,
+    """The KH 2048^2 workload at full size: one SSPRK3 step bitwise against the oracle,
     then 10 more device steps conserve mass, momentum and energy (periodic)."""
     cfg = problems.kh_config(2048)
     q0 = problems.kh_band(cfg)
