@@ -95,6 +95,14 @@ struct FormF32c {
         buf_store_f32(v.x, b.hi, voff, s_elem * kElem);
         buf_store_f32(v.y, b.lo, voff, s_elem * kElem);
     }
+    // direct-to-LDS load of an element's hi and lo words
+    __device__ static inline void dma(const Buf& b, uint32_t* hi, uint32_t* lo, uint32_t voff, uint32_t s_elem) {
+        buf_load_lds(b.hi, hi, voff, s_elem * kElem);
+        buf_load_lds(b.lo, lo, voff, s_elem * kElem);
+    }
+    __device__ static inline float2 from_words(uint32_t hi, uint32_t lo) {
+        return make_float2(__builtin_bit_cast(float, hi), __builtin_bit_cast(float, lo));
+    }
     __device__ static inline void put(const Planes& a, int idx, float2 v) {
         a.hi[idx] = v.x;
         a.lo[idx] = v.y;

@@ -129,6 +129,25 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
     T win[WIN][NV];
     const bool out_lane = lane >= 1 && lane <= kStripOut && c < g.ng + g.nx;
+
+    // Compensated form, plain instance: the stage-combine operands of the
+    // next row (Q^(s) and Q^n, hi and lo words) are loaded straight into LDS
+    // while this row is swept, instead of into 16 registers held across the
+    // sweeps.
+    constexpr bool kDmaOps = kBuf && Form::kEarlyOperands;
+    __shared__ uint32_t opnd[kDmaOps ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][hi, lo][lane]
+    auto issue_operands = [&](int jr) {
+        if constexpr (kDmaOps) {
+            const int row = pk + jr * xc;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                Form::dma(bqs, &opnd[0][v][0][0], &opnd[0][v][1][0], (uint32_t)c_ld * ES, (uint32_t)(v * plane + row));
+                if (Form::needs_qn(g.kind))
+                    Form::dma(bqn, &opnd[1][v][0][0], &opnd[1][v][1][0], (uint32_t)c_ld * ES,
+                              (uint32_t)(v * plane + row));
+            }
+        }
+    };
     unsigned int floor_hits = 0;
 
     auto commit_row = [&](int r, const Raw (&q)[NV]) {
@@ -216,6 +235,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
         if (j0 < j1) fetch_row(j0 + 1 + ORD, qa);
+        if (j0 < j1) issue_operands(j0);
     }
 
     double amax = 0.0;
@@ -238,7 +258,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 }
             }
         };
-        if constexpr (Form::kEarlyOperands) load_operands();
+        if constexpr (Form::kEarlyOperands && !kDmaOps) load_operands();
         wave_lds_sync();  // earlier ring reads complete before the slot is reused
         commit_row(j + 1 + ORD, qa);
         if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
@@ -273,6 +293,15 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 
+        if constexpr (kDmaOps) {  // this row's operands, loaded into LDS during the previous row
+            wait_vm_loads();
+            wave_lds_sync();
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                qs_c[v] = Form::from_words(opnd[0][v][0][lane], opnd[0][v][1][lane]);
+                qn_c[v] = Form::needs_qn(g.kind) ? Form::from_words(opnd[1][v][0][lane], opnd[1][v][1][lane]) : qs_c[v];
+            }
+        }
         if (out_lane) {
             if constexpr (!Form::kEarlyOperands) load_operands();
             Raw qo[NV];
@@ -297,6 +326,13 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             if (g.reduce_dt) {
                 const double a = Form::template signal_speed_nd<ND>((double)p.gamma, qo);
                 amax = a > amax ? a : amax;
+            }
+        }
+        if constexpr (kDmaOps) {
+            if (j + 1 < j1) {
+                wait_lds_ops();  // this row's operand words are read before the next row's overwrite them
+                wave_lds_sync();
+                issue_operands(j + 1);
             }
         }
     }

@@ -83,6 +83,19 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
 __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
 }
+// Direct-to-LDS load: lane t's dword at (voff + soff) lands in lds[t]
+// without passing through registers. The compiler does not track it: the
+// reader issues wait_vm_loads() first, and the LDS words it overwrites must
+// have been read (wait_lds_ops()) before it is issued.
+__device__ inline void buf_load_lds(__amdgpu_buffer_rsrc_t r, uint32_t* lds, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 4, (int)voff,
+                                             (int)soff, 0, 0);
+}
+// s_waitcnt vmcnt(0): every vector-memory operation of this wave has completed.
+__device__ inline void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt lgkmcnt(0): every LDS operation of this wave has completed.
+__device__ inline void wait_lds_ops() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
 __device__ inline float buf_load_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
