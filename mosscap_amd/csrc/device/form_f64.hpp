@@ -48,6 +48,14 @@ struct FormF64 {
     __device__ static inline void bput(const Buf& b, uint32_t voff, uint32_t s_elem, double v) {
         buf_store_f64(v, b.p, voff, s_elem * kElem);
     }
+    // direct-to-LDS load of an element as its low and high words
+    __device__ static inline void dma(const Buf& b, uint32_t* lo, uint32_t* hi, uint32_t voff, uint32_t s_elem) {
+        buf_load_lds(b.p, lo, voff, s_elem * kElem);
+        buf_load_lds(b.p, hi, voff + 4u, s_elem * kElem);
+    }
+    __device__ static inline double from_words(uint32_t lo, uint32_t hi) {
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    }
     __device__ static inline void put(const Planes& a, int idx, double v) { a.p[idx] = v; }
     __device__ static inline bool needs_qn(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 5; }
     __device__ static inline double negate(double v) { return -v; }

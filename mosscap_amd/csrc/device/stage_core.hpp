@@ -130,12 +130,12 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     T win[WIN][NV];
     const bool out_lane = lane >= 1 && lane <= kStripOut && c < g.ng + g.nx;
 
-    // Compensated form, plain instance: the stage-combine operands of the
-    // next row (Q^(s) and Q^n, hi and lo words) are loaded straight into LDS
-    // while this row is swept, instead of into 16 registers held across the
-    // sweeps.
-    constexpr bool kDmaOps = kBuf && Form::kEarlyOperands;
-    __shared__ uint32_t opnd[kDmaOps ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][hi, lo][lane]
+    // Plain instance: the stage-combine operands of the next row (Q^(s) and
+    // Q^n, two words per value) are loaded straight into LDS while this row
+    // is swept: no registers held across the sweeps (the compensated form
+    // held 16) and no load latency at the combine (the FP64 form's).
+    constexpr bool kDmaOps = kBuf;
+    __shared__ uint32_t opnd[kDmaOps ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][word][lane]
     auto issue_operands = [&](int jr) {
         if constexpr (kDmaOps) {
             const int row = pk + jr * xc;
@@ -303,7 +303,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             }
         }
         if (out_lane) {
-            if constexpr (!Form::kEarlyOperands) load_operands();
+            if constexpr (!Form::kEarlyOperands && !kDmaOps) load_operands();
             Raw qo[NV];
             T srcv[NV];
             if (Extras && p.gravity) {
