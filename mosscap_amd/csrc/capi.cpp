@@ -425,6 +425,7 @@ int mh_time_step(mh_ctx* ctx, double dt) {
     if (e == hipSuccess) e = launch_clock_set_dt(ctx->clock, dt, ctx->stream);
     int nc = ctx->cur;
     if (e == hipSuccess) nc = ctx->enqueue_step(ctx->cur, false, e);
+    if (e == hipSuccess) e = ctx->fill_final(nc);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_time_step");
     ctx->cur = nc;
@@ -471,6 +472,7 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
         }
         ctx->steps += 1;
     }
+    if (e == hipSuccess && nsteps > 0) e = ctx->fill_final(ctx->cur);
     Clock h{};
     if (e == hipSuccess) e = hipMemcpyAsync(&h, ctx->clock, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

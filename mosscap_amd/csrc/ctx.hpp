@@ -214,15 +214,39 @@ struct mh_ctx {
     // (TimeStepping.cpp:52, 129, 226), so for stage 0 (in == qn_k) the updated
     // input goes to the third buffer, which no stage of the step has written
     // yet, and Q^n keeps the values it had.
-    hipError_t stage(int in, int qn_k, int out, int kind, int reduce) {
+    // With `fill` false and virtual_ghosts(), the output's ghosts are left
+    // stale: the next stage reads ghost rows and columns from their sources.
+    hipError_t stage(int in, int qn_k, int out, int kind, int reduce, bool fill = true) {
         const int src = !tab_on ? in : (in == qn_k ? 3 - in - out : in);
         hipError_t e = timed(0, [&] {
             hipError_t r = tab_on ? tab_update(in, src) : hipSuccess;
             if (r == hipSuccess) r = stage_kernel(src, qn_k, out, kind, reduce);
             return r;
         });
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess || (!fill && virtual_ghosts())) return e;
         return ghost(src, out, reduce);
+    }
+
+    // Compensated form on 2-D grids whose faces are all periodic or zerograd,
+    // with no user_bc hook, no tracer fields and no tabulated EOS (its update
+    // runs over the ghosts of each stage's input): every ghost is a copy of
+    // an interior cell, so the stage kernel reads ghost rows and columns from
+    // their sources (StageGrid::virt) and a step needs one ghost fill, at its
+    // end, instead of one per stage. Leaving the ghosts out of the CFL max
+    // changes nothing (they copy interior values). The FP64 drop-in keeps the
+    // reference's fill after every stage (Hydro.cpp, TimeStepping.cpp), so its
+    // stage sequence, and the adjoint that recomputes it, stay the reference's.
+    bool virtual_ghosts() const {
+        const auto copy_bc = [](int32_t b) {
+            return b == (int32_t)BoundaryType::Periodic || b == (int32_t)BoundaryType::ZeroGrad;
+        };
+        return fp32c() && nd == 2 && !user_fn && ntr == 0 && !tab_on && copy_bc(bdry.xs) && copy_bc(bdry.xe) &&
+               copy_bc(bdry.ys) && copy_bc(bdry.ye);
+    }
+    void set_virt(StageGrid& g) const {
+        g.virt = virtual_ghosts() ? 1 : 0;
+        g.xper = bdry.xs == (int32_t)BoundaryType::Periodic;
+        g.yper = bdry.ys == (int32_t)BoundaryType::Periodic;
     }
 
     // 3-D march grid (the interior fold of the CFL max with `reduce`)
@@ -242,6 +266,7 @@ struct mh_ctx {
                              gravity, grav[0], grav[1]};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+                set_virt(g);
                 e = launch_stage2d_f32c(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
             } else if (nd == 3) {
@@ -255,6 +280,7 @@ struct mh_ctx {
                             frx64, fry64, gravity, grav[0], grav[1]};
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+                set_virt(g);
                 e = launch_stage2d_f64(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
             } else if (nd == 3) {
@@ -292,6 +318,8 @@ struct mh_ctx {
 
     // All stages of one step starting from `cur`; returns the new cur.
     // With a user_bc hook the reduction is a separate pass after the hook.
+    // With virtual_ghosts() no stage fills ghosts: the caller fills those of
+    // the last step's result (fill_final).
     int enqueue_step(int k, bool reduce_last, hipError_t& e) {
         const int ns = num_stages(sch.stepper);
         const bool fused = reduce_last && !user_fn;
@@ -299,13 +327,16 @@ struct mh_ctx {
         for (int s = 0; s < ns; ++s) {
             last_in = in;
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
-            e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0);
+            e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0, false);
             if (e != hipSuccess) return k;
             in = out;
         }
         if (reduce_last && !fused) e = cfl_all(in);
         return in;
     }
+
+    // ghosts of buffer k after steps run with virtual ghosts
+    hipError_t fill_final(int k) { return virtual_ghosts() ? ghost(k, k, 0) : hipSuccess; }
 
     // kernel arguments changed: graphs captured before must be rebuilt
     void drop_graphs() {

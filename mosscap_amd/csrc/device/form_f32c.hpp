@@ -83,8 +83,11 @@ struct FormF32c {
     struct Buf {
         __amdgpu_buffer_rsrc_t hi, lo;
     };
-    // the march loads Q^(s) and Q^n for the combine at the top of the row
-    // (their latency hides behind the sweeps; 4 waves per SIMD either way)
+    // 2-D grids with copy-type faces read ghost rows and columns from their
+    // sources instead of filling them after every stage (StageGrid::virt)
+    static constexpr bool kVirtualGhosts = true;
+    // the instance without buffer resources loads Q^(s) and Q^n for the
+    // combine at the top of the row (their latency hides behind the sweeps)
     static constexpr bool kEarlyOperands = true;
     static constexpr uint32_t kElem = 4;
     __device__ static inline Buf buf(const Planes& a) { return Buf{make_rsrc(a.hi), make_rsrc(a.lo)}; }

@@ -37,8 +37,10 @@ struct FormF64 {
     struct Buf {
         __amdgpu_buffer_rsrc_t p;
     };
-    // the march loads Q^(s) and Q^n for the combine just before it (fewer
-    // registers live across the sweeps: 2 waves per SIMD instead of 1)
+    // the drop-in fills ghosts after every stage, as the reference does
+    static constexpr bool kVirtualGhosts = false;
+    // the instance without buffer resources loads Q^(s) and Q^n for the
+    // combine just before it (fewer registers live across the sweeps)
     static constexpr bool kEarlyOperands = false;
     static constexpr uint32_t kElem = 8;
     __device__ static inline Buf buf(const Planes& a) { return Buf{make_rsrc(a.p)}; }

@@ -76,6 +76,16 @@ __host__ __device__ inline int ghost_source(int g, int dims, int ng, BoundaryTyp
     }
 }
 
+// Source index of index i of an axis with n interior cells and ng ghosts on
+// each side when its ghosts are copies: ghost_source() for Periodic and
+// ZeroGrad; interior indices map to themselves. Readers of a state whose
+// ghosts were not filled use it in place of the fill.
+__host__ __device__ inline int copy_ghost_source(int i, int n, int ng, bool periodic) {
+    if (i < ng) return periodic ? i + n : ng;
+    if (i >= ng + n) return periodic ? i - n : ng + n - 1;
+    return i;
+}
+
 // Modification after the copy (Boundaries.hpp:83-115).
 template <class Form, int ND, int NV, int Axis>
 __device__ inline void ghost_modify(typename Form::raw (&q)[NV], BoundaryType b, bool start, const double* cvals) {

@@ -53,7 +53,13 @@ struct StageGrid {
     int32_t kind;        // StageKind
     int32_t reduce_dt;   // fold the new state into the CFL max
     int32_t zc = 1, nz = 1;  // 3-D: planes (ghosts included) and interior planes
+    // 2-D, every face periodic or zerograd: ghost rows and columns are read
+    // from their source cells (their fill is a plain copy), so the stage
+    // needs no ghost fill of its input. xper / yper: periodic along x / y,
+    // else zerograd.
+    int32_t virt = 0, xper = 0, yper = 0;
 };
+
 
 // The x-y march of one plane. ND = 2: the whole 2-D stage. ND = 3: one
 // interior z-plane per (strip, span, plane) workgroup; the z flux difference
@@ -85,7 +91,11 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int j0 = g.ng + span * g.span_h;
     const int j1 = min(j0 + g.span_h, g.ng + g.ny);
     const int c = i0 - 1 + lane;
-    const int c_ld = min(c, g.xc - 1);
+    // ghosts read from their sources (StageGrid::virt; compiled for the forms that use it)
+    constexpr bool kVirt = ND == 2 && Form::kVirtualGhosts;
+    const int c_ld = (kVirt && g.virt) ? copy_ghost_source(min(c, g.xc - 1), g.nx, g.ng, g.xper) : min(c, g.xc - 1);
+    // source row of row r (wave-uniform)
+    auto src_row = [&](int r) { return (kVirt && g.virt) ? copy_ghost_source(r, g.ny, g.ng, g.yper) : r; };
     const int xc = g.xc;
     // variables are [var][k][j][i]: var stride of the whole grid, then this plane's offset
     const int plane = g.xc * g.yc * (ND > 2 ? g.zc : 1);
@@ -161,13 +171,18 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 wave_lds_sync();  // the previous batch's reads are done
                 const int hb_idx = lane % NH;  // this lane's halo column
                 const int hb_row = lane / NH;  // and row offset in the batch
-                const int hb_col = min(i0 - 1 - ORD + (hb_idx < ORD ? hb_idx : kWave + hb_idx), g.xc - 1);
+                int hb_col = min(i0 - 1 - ORD + (hb_idx < ORD ? hb_idx : kWave + hb_idx), g.xc - 1);
                 const int rr = r + hb_row;
                 if (hb_row < HB && rr <= r_last) {
+                    int rs = rr;
+                    if (kVirt && g.virt) {
+                        hb_col = copy_ghost_source(hb_col, g.nx, g.ng, g.xper);
+                        rs = copy_ghost_source(rr, g.ny, g.ng, g.yper);
+                    }
                     Raw qh[NV];
                     // the lane's row goes in the vector offset (the scalar one is wave-uniform)
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) qh[v] = load_qs(rr * xc + hb_col, v * plane + pk);
+                    for (int v = 0; v < NV; ++v) qh[v] = load_qs(rs * xc + hb_col, v * plane + pk);
                     T wh[NV];
                     Form::template to_prim_nd<ND>(p, qh, wh);
 #pragma unroll
@@ -200,7 +215,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         for (int v = 0; v < NV; ++v) win[WIN - 1][v] = w[v];
     };
     auto fetch_row = [&](int r, Raw (&q)[NV]) {
-        const int base = pk + r * xc;
+        const int base = pk + src_row(r) * xc;
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = load_qs(c_ld, v * plane + base);
     };
@@ -241,8 +256,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     double amax = 0.0;
 
     for (int j = j0; j < j1; ++j) {
-        // Stage-combine operands of this row: loaded here or just before the
-        // combine (Form::kEarlyOperands; latency hiding against registers).
+        // Stage-combine operands of this row: from LDS in the plain instance
+        // (loaded during the previous row); otherwise loaded here or just
+        // before the combine (Form::kEarlyOperands).
         const int row = pk + j * xc;
         Raw qs_c[NV], qn_c[NV];
         auto load_operands = [&] {
