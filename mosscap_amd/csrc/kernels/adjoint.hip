@@ -69,8 +69,15 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
 
     const AdjGrid& g = a.g;
     const int t = threadIdx.x;
-    const int seg = blockIdx.x % a.nseg;
-    const int line = blockIdx.x / a.nseg;
+    // Logical block ids run contiguously within each XCD (xcd_remap), so
+    // neighbouring lines share an L2. The x sweep keeps a row's segments
+    // adjacent (they share halo cells); the y sweep takes columns fastest, so
+    // the 16 neighbouring columns that share each cache line of a row run
+    // together.
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t nlines = gridDim.x / (uint32_t)a.nseg;
+    const int seg = Axis == 0 ? (int)(lb % (uint32_t)a.nseg) : (int)(lb / nlines);
+    const int line = Axis == 0 ? (int)(lb / (uint32_t)a.nseg) : (int)(lb % nlines);
     const int len = Axis == 0 ? g.xc : g.yc;
     const int nax = Axis == 0 ? g.nx : g.ny;
     const int p = seg * OUT - HALO + t;  // position along the line
