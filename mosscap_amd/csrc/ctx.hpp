@@ -227,26 +227,28 @@ struct mh_ctx {
         return ghost(src, out, reduce);
     }
 
-    // Compensated form on 2-D grids whose faces are all periodic or zerograd,
-    // with no user_bc hook, no tracer fields and no tabulated EOS (its update
-    // runs over the ghosts of each stage's input): every ghost is a copy of
-    // an interior cell, so the stage kernel reads ghost rows and columns from
-    // their sources (StageGrid::virt) and a step needs one ghost fill, at its
-    // end, instead of one per stage. Leaving the ghosts out of the CFL max
-    // changes nothing (they copy interior values). The FP64 drop-in keeps the
+    // Compensated form on 2-D and 3-D grids whose faces are all periodic or
+    // zerograd, with no user_bc hook, no tracer fields and no tabulated EOS
+    // (its update runs over the ghosts of each stage's input): every ghost is
+    // a copy of an interior cell, so the stage kernels read ghost rows,
+    // columns and planes from their sources (StageGrid::virt) and a step
+    // needs one ghost fill, at its end, instead of one per stage. Leaving the
+    // ghosts out of the CFL max changes nothing (they copy interior values).
+    // The FP64 drop-in keeps the
     // reference's fill after every stage (Hydro.cpp, TimeStepping.cpp), so its
     // stage sequence, and the adjoint that recomputes it, stay the reference's.
     bool virtual_ghosts() const {
         const auto copy_bc = [](int32_t b) {
             return b == (int32_t)BoundaryType::Periodic || b == (int32_t)BoundaryType::ZeroGrad;
         };
-        return fp32c() && nd == 2 && !user_fn && ntr == 0 && !tab_on && copy_bc(bdry.xs) && copy_bc(bdry.xe) &&
-               copy_bc(bdry.ys) && copy_bc(bdry.ye);
+        return fp32c() && nd >= 2 && !user_fn && ntr == 0 && !tab_on && copy_bc(bdry.xs) && copy_bc(bdry.xe) &&
+               copy_bc(bdry.ys) && copy_bc(bdry.ye) && (nd < 3 || (copy_bc(bdry.zs) && copy_bc(bdry.ze)));
     }
     void set_virt(StageGrid& g) const {
         g.virt = virtual_ghosts() ? 1 : 0;
         g.xper = bdry.xs == (int32_t)BoundaryType::Periodic;
         g.yper = bdry.ys == (int32_t)BoundaryType::Periodic;
+        g.zper = bdry.zs == (int32_t)BoundaryType::Periodic;
     }
 
     // 3-D march grid (the interior fold of the CFL max with `reduce`)
@@ -254,6 +256,7 @@ struct mh_ctx {
         StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
         g.zc = sz.zc;
         g.nz = sz.zc - 2 * sz.ng;
+        set_virt(g);
         return g;
     }
 

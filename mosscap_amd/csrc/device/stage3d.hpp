@@ -26,6 +26,7 @@ struct ZSweepGrid {
     int32_t xc, yc, zc, ng;
     int32_t nx, ny, nz;
     int32_t kchunk;  // interior planes per thread (blockIdx.y selects the chunk)
+    int32_t virt = 0, zper = 0;  // ghost planes read from their sources (StageGrid::virt)
 };
 
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
@@ -51,9 +52,10 @@ __device__ inline void zsweep_column(const ZSweepGrid& g, const typename Form::P
 
     T win[WIN][NV];
     auto load = [&](int k, T (&w)[NV]) {
+        const int ks = (Form::kVirtualGhosts && g.virt) ? copy_ghost_source(k, g.nz, g.ng, g.zper) : k;
         Raw q[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) q[v] = Form::get(p.qs, v * vs + k * pl + base);
+        for (int v = 0; v < NV; ++v) q[v] = Form::get(p.qs, v * vs + ks * pl + base);
         Form::template to_prim_nd<ND>(p, q, w);
     };
     auto recon = [&](T (&wl)[NV], T (&wr)[NV]) {

@@ -53,11 +53,11 @@ struct StageGrid {
     int32_t kind;        // StageKind
     int32_t reduce_dt;   // fold the new state into the CFL max
     int32_t zc = 1, nz = 1;  // 3-D: planes (ghosts included) and interior planes
-    // 2-D, every face periodic or zerograd: ghost rows and columns are read
-    // from their source cells (their fill is a plain copy), so the stage
-    // needs no ghost fill of its input. xper / yper: periodic along x / y,
-    // else zerograd.
-    int32_t virt = 0, xper = 0, yper = 0;
+    // Every face periodic or zerograd: ghost rows, columns (and, for the z
+    // sweep, planes) are read from their source cells (their fill is a plain
+    // copy), so the stage needs no ghost fill of its input. xper / yper /
+    // zper: periodic along that axis, else zerograd.
+    int32_t virt = 0, xper = 0, yper = 0, zper = 0;
 };
 
 
@@ -92,7 +92,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int j1 = min(j0 + g.span_h, g.ng + g.ny);
     const int c = i0 - 1 + lane;
     // ghosts read from their sources (StageGrid::virt; compiled for the forms that use it)
-    constexpr bool kVirt = ND == 2 && Form::kVirtualGhosts;
+    constexpr bool kVirt = Form::kVirtualGhosts;
     const int c_ld = (kVirt && g.virt) ? copy_ghost_source(min(c, g.xc - 1), g.nx, g.ng, g.xper) : min(c, g.xc - 1);
     // source row of row r (wave-uniform)
     auto src_row = [&](int r) { return (kVirt && g.virt) ? copy_ghost_source(r, g.ny, g.ng, g.yper) : r; };

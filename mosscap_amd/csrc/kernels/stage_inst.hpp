@@ -175,7 +175,9 @@ hipError_t launch_stage3d_impl(const Schemes& s, const StageGrid& g, const typen
     };
     // z sweep: chunks of at most 32 planes per thread, columns of the interior across the grid's x
     const int chunks = (g.nz + 31) / 32;
-    const ZSweepGrid zg{g.xc, g.yc, g.zc, g.ng, g.nx, g.ny, g.nz, (g.nz + chunks - 1) / chunks};
+    ZSweepGrid zg{g.xc, g.yc, g.zc, g.ng, g.nx, g.ny, g.nz, (g.nz + chunks - 1) / chunks};
+    zg.virt = g.virt;
+    zg.zper = g.zper;
     const int cols = g.nx * g.ny;
     hipLaunchKernelGGL(pick_scheme<Form>(s, zpick), dim3((unsigned)((cols + 127) / 128), (unsigned)chunks), dim3(128),
                        0, st, zg, p, dz);
