@@ -83,18 +83,32 @@ struct mh_batch {
             }
     }
 
+    // Compensated form with periodic or zerograd faces: the persistent stage
+    // reads ghosts from their source cells, so the stages of a step fill no
+    // ghosts and mh_batch_run fills the final state's once (the ghosts would
+    // add nothing to the per-problem CFL max: they copy interior values).
+    bool virtual_ghosts() const {
+        const auto copy_bc = [](int32_t x) {
+            return x == (int32_t)BoundaryType::Periodic || x == (int32_t)BoundaryType::ZeroGrad;
+        };
+        return fp32c() && copy_bc(bdry.xs) && copy_bc(bdry.xe);
+    }
+
     // one stage for every problem: persistent (mode 0) or one launch per problem (mode 1)
     hipError_t stage(int mode, int in, int qn_k, int out, int kind, int reduce) {
         hipError_t e = hipSuccess;
         const BatchArrays b = arrays();
         if (mode == 0) {
-            const BatchGrid g{ng, nitems, nprob, kind, reduce};
+            BatchGrid g{ng, nitems, nprob, kind, reduce, 0, 0};
+            g.virt = virtual_ghosts() ? 1 : 0;
+            g.per = bdry.xs == (int32_t)BoundaryType::Periodic;
             const int grid = std::min(waves > 0 ? waves : cus * 12, nitems);
             if (fp32c()) {
                 FormF32c::Ptrs p{p32(in), p32(qn_k), p32(out), d_dt, 0.0, gamma, gamma - 1.0, d_amax,
                                  floor_hits, opt.rho_floor, opt.p_floor};
                 e = launch_batch_stage_f32c(sch, g, grid, p, b, stream);
-                if (e == hipSuccess) e = launch_batch_ghost_f32c(ghost_grid(reduce), nprob, p32(in), p32(out), b, stream);
+                if (e == hipSuccess && !g.virt)
+                    e = launch_batch_ghost_f32c(ghost_grid(reduce), nprob, p32(in), p32(out), b, stream);
             } else {
                 FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), d_dt, 0.0, gamma, d_amax, nullptr};
                 e = launch_batch_stage_f64(sch, g, grid, p, b, stream);
@@ -343,6 +357,8 @@ int mh_batch_run(mh_batch* b, int64_t nsteps, double max_cfl, double max_time, i
         }
         b->steps += 1;
     }
+    if (e == hipSuccess && mode == 0 && nsteps > 0 && b->virtual_ghosts())  // the final state's ghosts
+        e = launch_batch_ghost_f32c(b->ghost_grid(0), b->nprob, b->p32(b->cur), b->p32(b->cur), arr, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     return e == hipSuccess ? MH_OK : b->hip_fail(e, "mh_batch_run");
 }

@@ -25,6 +25,7 @@ struct BatchArrays {
 
 struct BatchGrid {
     int32_t ng, nitems, nprob, kind, reduce_dt;
+    int32_t virt, per;  // ghosts read from their sources (periodic `per`, else zerograd); no fill per stage
 };
 
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
@@ -42,7 +43,7 @@ __device__ inline void batch_stage(const BatchGrid& g, const typename Form::Ptrs
         const T dtdx = dt / (T)b.dx[pr];
         double amax = 0.0;
         stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, b.base[pr], xc, g.ng, xc - 2 * g.ng, i0, dtdx,
-                                         dt, lds, amax, hits);
+                                         dt, lds, amax, hits, Form::kVirtualGhosts ? g.virt : 0, g.per != 0);
         if (g.reduce_dt) {
             const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
             if (lane == 0) atomicMax(&b.amax[pr], (unsigned long long)m);

@@ -372,7 +372,7 @@ template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
 __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, int reduce_dt, int base, int xc,
                                        int ng, int nx, int i0, const typename Form::real dtdx,
                                        const typename Form::real dt, typename Form::real* lds_row /*[3][64+2R]*/,
-                                       double& amax, unsigned int& floor_hits) {
+                                       double& amax, unsigned int& floor_hits, int virt = 0, bool per = false) {
     using T = typename Form::real;
     using Raw = typename Form::raw;
     constexpr int NV = 3;  // rho, mx, E
@@ -380,10 +380,12 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
     constexpr int ROW = kWave + 2 * ORD;
     const int lane = threadIdx.x & 63;
     const int c = i0 - 1 + lane;
-    const int c_ld = min(c, xc - 1);
+    // virt: ghosts are read from their sources (periodic `per`, else zerograd)
+    const int c_ld = virt ? copy_ghost_source(min(c, xc - 1), nx, ng, per) : min(c, xc - 1);
     const bool halo_lane = lane < 2 * ORD;
     const int h_entry = lane < ORD ? lane : kWave + lane;
-    const int h_col = min(i0 - 1 - ORD + h_entry, xc - 1);
+    const int h_col = virt ? copy_ghost_source(min(i0 - 1 - ORD + h_entry, xc - 1), nx, ng, per)
+                           : min(i0 - 1 - ORD + h_entry, xc - 1);
     const EosT<T> eos = make_eos<T>((double)p.gamma);
 
     Raw q[NV], qn[NV], qh[NV];
