@@ -13,12 +13,15 @@ tests/test_oracle_*.py for the pins.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "_build" / "libmh_oracle.so"
+# MH_ORACLE_LIB points the checks at another build of the oracle (a build with a
+# planted mistake shows which pins catch it)
+LIB_PATH = Path(os.environ.get("MH_ORACLE_LIB", _HERE / "_build" / "libmh_oracle.so"))
 
 RECON = {"fog": 0, "muscl": 1, "ppm": 2, "weno5z": 3}
 LIMITER = {"vanleer": 0, "monotonizedcentral": 1, "minmod": 2}
