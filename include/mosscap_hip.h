@@ -124,7 +124,12 @@ int mh_stage(mh_ctx* ctx, int32_t stage, double dt);
  * the previous step's last stage), advances and does the output-time
  * bookkeeping, for nsteps steps (steps past max_time have dt = 0). *time and
  * *prev_output_time are read and updated. With use_graph, steps replay a
- * captured HIP graph. */
+ * captured HIP graph. The first dt of a call comes from a CFL pass over the
+ * state, except right after another mh_run, whose last stage left the
+ * current state's maximum signal speed behind (every other call that changes
+ * the state clears it). A caller that writes the state through
+ * mh_device_state outside a user_bc hook calls mh_fill_bcs afterwards, as the
+ * reference's setup does, which clears it too. */
 int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max_time, double* prev_output_time,
            double delta, int32_t use_graph);
 
@@ -151,7 +156,9 @@ int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, doub
  * events, keeps the fastest for this context and returns it in *best_span
  * (and the per-candidate milliseconds per step in ms_out when non-NULL). The
  * state, the tabulated EOS's fields and the floor-hit count are restored
- * afterwards; a user_bc hook runs during the trials like in any step. */
+ * afterwards; the record of the last stage's input is cleared (the trials
+ * overwrote that buffer), so mh_eos_fields(which = 1) then reads the current
+ * state. A user_bc hook runs during the trials like in any step. */
 int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* best_span, double* ms_out);
 
 int mh_get_stats(mh_ctx* ctx, mh_stats* out);
@@ -185,9 +192,10 @@ int mh_set_gravity(mh_ctx* ctx, int32_t enable, double gx, double gy, double gz)
  * fraction y and temperature T (bisection on T, Saha equation for y) of the
  * state the reference's update_eos last saw: which = 1 takes the input of the
  * most recent stage (update_eos runs at the start of each stage's fluxes,
- * Hydro.cpp:325-328), which = 0 the current state. The hydro step does not use
- * them. An inverted temperature bracket (the reference aborts) returns
- * MH_ERR_INVALID. */
+ * Hydro.cpp:325-328), which = 0 the current state. The ghosts of that input
+ * hold the values the stage read (a step that read ghosts from their source
+ * cells fills them first). The hydro step does not use them. An inverted
+ * temperature bracket (the reference aborts) returns MH_ERR_INVALID. */
 int mh_eos_fields(mh_ctx* ctx, int32_t which, double avg_mass, int32_t ionisation_e, double* y, double* T);
 
 typedef int (*mh_user_bc_fn)(mh_ctx* ctx, void* user);

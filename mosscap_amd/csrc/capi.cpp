@@ -190,6 +190,7 @@ int mh_set_user_bc(mh_ctx* ctx, mh_user_bc_fn fn, void* user) {
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
     ctx->user_fn = fn;
     ctx->user_data = fn ? user : nullptr;
+    ctx->amax_valid = false;
     return MH_OK;
 }
 
@@ -217,7 +218,12 @@ int mh_eos_fields(mh_ctx* ctx, int32_t which, double avg_mass, int32_t ionisatio
     const int k = (which == 1 && ctx->last_in >= 0) ? ctx->last_in : ctx->cur;
     const LteParams prm{ctx->gamma, 1.0 / avg_mass, ionisation_e ? 1 : 0};
     int flag = 0;
-    e = hipMemsetAsync(ctx->eos_flag, 0, sizeof(int), ctx->stream);
+    // Steps with ghosts read from their sources leave the ghosts of the
+    // intermediate stage inputs unfilled; the fields cover every cell, so
+    // fill them here (plain copies of interior cells, the values those
+    // stages read).
+    e = (k != ctx->cur && ctx->virtual_ghosts()) ? ctx->ghost(k, k, 0) : hipSuccess;
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->eos_flag, 0, sizeof(int), ctx->stream);
     if (e == hipSuccess) {
         e = ctx->fp32c() ? launch_lte_eos_f32c(ctx->nd, ctx->p32(k), (int)ctx->ncells, prm, ctx->eos_y, ctx->eos_T,
                                                ctx->eos_flag, ctx->stream)
@@ -258,6 +264,7 @@ static int upload_impl(mh_ctx* c, const double* q, hipMemcpyKind kind) {
     } else {
         if (e == hipSuccess) e = hipMemcpyAsync(c->q64[c->cur], q, bytes, kind, c->stream);
     }
+    c->amax_valid = false;
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "mh_upload");
     c->next_stage = 0;
@@ -288,12 +295,15 @@ int mh_download_device(mh_ctx* ctx, double* q_dev) { return download_impl(ctx, q
 int mh_fill_bcs(mh_ctx* ctx) {
     if (!ctx) return MH_ERR_INVALID;
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;  // a hook may change ghost values; state written through mh_device_state
     if (e == hipSuccess) e = ctx->ghost(ctx->cur, ctx->cur, 0);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_fill_bcs");
 }
 
-static hipError_t write_clock(mh_ctx* c, double max_cfl, double time, double max_time, double prev, double delta) {
+// Writes the clock's fields; clears the CFL slots unless `keep_max`.
+static hipError_t write_clock(mh_ctx* c, double max_cfl, double time, double max_time, double prev, double delta,
+                              bool keep_max = false) {
     Clock h{};
     h.time = time;
     h.dt = 0.0;
@@ -304,7 +314,7 @@ static hipError_t write_clock(mh_ctx* c, double max_cfl, double time, double max
     h.dx = c->dx;
     h.step = c->steps;
     const hipError_t e = hipMemcpyAsync(c->clock, &h, sizeof(Clock), hipMemcpyHostToDevice, c->stream);
-    return e == hipSuccess ? launch_clock_reset_max(c->clock, c->stream) : e;
+    return (e == hipSuccess && !keep_max) ? launch_clock_reset_max(c->clock, c->stream) : e;
 }
 
 // The tabulated EOS's abort condition (TabulatedLteH.hpp:140-142) after a
@@ -329,6 +339,7 @@ int mh_set_tabulated_eos(mh_ctx* ctx, const mh_lte_table* t) {
     if (ctx->tab_neg) hipFree(ctx->tab_neg);
     ctx->tab_neg = nullptr;
     ctx->tab_on = 0;
+    ctx->amax_valid = false;
     ctx->drop_graphs();
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_set_tabulated_eos");
     if (!t) return MH_OK;
@@ -361,6 +372,7 @@ int mh_tabulated_eos_update(mh_ctx* ctx) {
     if (!ctx->tab_on) return ctx->fail(MH_ERR_INVALID, "no tabulated EOS is installed");
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;
     if (e == hipSuccess) e = ctx->tab_update(ctx->cur, ctx->cur);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_tabulated_eos_update");
@@ -383,6 +395,7 @@ int mh_compute_dt(mh_ctx* ctx, double max_cfl, double time, double max_time, dou
                   double delta, double* dt_out) {
     if (!ctx || !dt_out) return MH_ERR_INVALID;
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;
     if (e == hipSuccess) e = write_clock(ctx, max_cfl, time, max_time, prev_output_time, delta);
     if (e == hipSuccess) e = ctx->cfl_all(ctx->cur);
     if (e == hipSuccess) e = launch_clock_finalize(ctx->clock, 0, ctx->stream);
@@ -400,6 +413,7 @@ int mh_stage(mh_ctx* ctx, int32_t stage, double dt) {
     if (stage < 0 || stage >= ns) return ctx->fail(MH_ERR_INVALID, "stage out of range for the time stepper");
     if (stage != ctx->next_stage) return ctx->fail(MH_ERR_INVALID, "stages must run in order starting at 0");
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;
     if (stage == 0) ctx->qn = ctx->cur;
     const int in = ctx->cur;
     const int out = (in == ctx->qn) ? (ctx->qn + 1) % 3 : 3 - in - ctx->qn;
@@ -422,6 +436,7 @@ int mh_time_step(mh_ctx* ctx, double dt) {
     if (!ctx) return MH_ERR_INVALID;
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is already in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;
     if (e == hipSuccess) e = launch_clock_set_dt(ctx->clock, dt, ctx->stream);
     int nc = ctx->cur;
     if (e == hipSuccess) nc = ctx->enqueue_step(ctx->cur, false, e);
@@ -440,8 +455,13 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
     if (!ctx || !time || !prev_output_time || nsteps < 0) return MH_ERR_INVALID;
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is already in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
-    if (e == hipSuccess) e = write_clock(ctx, max_cfl, *time, max_time, *prev_output_time, delta);
-    if (e == hipSuccess) e = ctx->cfl_all(ctx->cur);
+    // Steady state: the previous mh_run's last stage left the max signal
+    // speed of the current state in the CFL slots, so only the clock's
+    // fields are written; otherwise a CFL pass over the state fills them.
+    const bool keep_max = ctx->amax_valid;
+    ctx->amax_valid = false;
+    if (e == hipSuccess) e = write_clock(ctx, max_cfl, *time, max_time, *prev_output_time, delta, keep_max);
+    if (e == hipSuccess && !keep_max) e = ctx->cfl_all(ctx->cur);
     for (int64_t s = 0; s < nsteps && e == hipSuccess; ++s) {
         const int k = ctx->cur;
         if (use_graph && !ctx->user_fn) {
@@ -478,6 +498,7 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_run");
     if (const int rc = tab_check(ctx); rc != MH_OK) return rc;
+    ctx->amax_valid = true;
     *time = h.time;
     *prev_output_time = h.prev_output_time;
     ctx->time = h.time;
@@ -564,7 +585,6 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
     if (e == hipSuccess)
         e = hipMemcpyAsync(&keep_hits, ctx->floor_hits, sizeof(keep_hits), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    const int old_last_in = ctx->last_in;
     const int old_span = ctx->span_h;
     const int64_t old_steps = ctx->steps;
     double best = 1e300;
@@ -612,7 +632,9 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
     if (e == hipSuccess)
         e = hipMemcpyAsync(ctx->floor_hits, &keep_hits, sizeof(keep_hits), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    ctx->last_in = old_last_in;
+    // the trial steps overwrote every buffer but `start`: the stage-input
+    // record is void, and mh_eos_fields(which = 1) now reads the current state
+    ctx->last_in = -1;
     if (a) hipEventDestroy(a);
     if (b) hipEventDestroy(b);
     if (keep_a) hipFree(keep_a);
@@ -623,6 +645,7 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
     }
     ctx->steps = old_steps;
     ctx->cur = start;
+    ctx->amax_valid = false;  // the slots hold a trial state's maximum
     if (e != hipSuccess) {
         ctx->span_h = old_span;
         return ctx->hip_fail(e, "mh_autotune");

@@ -53,6 +53,7 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
     if (ctx->sz.ng > 8) return ctx->fail(MH_ERR_UNSUPPORTED, "the adjoint supports at most 8 ghost cells");
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;  // the clock's CFL slots are reused here
     ctx->free_adjoint();
     const int nck = (int)((nsteps + checkpoint_every - 1) / checkpoint_every);
     const size_t bytes = (size_t)ctx->nvals * sizeof(double);
@@ -91,6 +92,7 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     const int64_t nsteps = (int64_t)ctx->adj_dts.size();
     if (nsteps == 0 || ctx->ckpt.empty()) return ctx->fail(MH_ERR_INVALID, "run mh_adjoint_forward first");
     hipError_t e = hipSetDevice(ctx->device);
+    ctx->amax_valid = false;  // the clock's CFL slots are reused here
     const int ns = num_stages(ctx->sch.stepper);
     const int every = ctx->adj_every;
     const size_t nv = (size_t)ctx->nvals;

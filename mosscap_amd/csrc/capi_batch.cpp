@@ -386,21 +386,28 @@ int mh_batch_autotune(mh_batch* b, int32_t steps, const int32_t* waves_per_cu, i
     const size_t bytes = (size_t)b->nvals * (b->fp32c() ? sizeof(float) : sizeof(double));
     void* keep_a = nullptr;
     void* keep_b = nullptr;
-    double* keep_t = nullptr;
+    double* keep_t = nullptr;  // per-problem time, then last dt
+    unsigned int keep_hits = 0;
+    const size_t tbytes = (size_t)b->nprob * sizeof(double);
     if (e == hipSuccess) e = hipMalloc(&keep_a, bytes);
     if (e == hipSuccess && b->fp32c()) e = hipMalloc(&keep_b, bytes);
-    if (e == hipSuccess) e = hipMalloc(&keep_t, b->nprob * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&keep_t, 2 * tbytes);
     const int start = b->cur;
+    // what the trial runs change: the state, the per-problem clocks and the floor-hit count
     auto stash = [&](bool restore) -> hipError_t {
+        const auto copy = [&](void* live, void* keep, size_t n) {
+            return restore ? hipMemcpyAsync(live, keep, n, hipMemcpyDeviceToDevice, b->stream)
+                           : hipMemcpyAsync(keep, live, n, hipMemcpyDeviceToDevice, b->stream);
+        };
         void* d0 = b->fp32c() ? (void*)b->qhi[start] : (void*)b->q64[start];
-        hipError_t r = restore ? hipMemcpyAsync(d0, keep_a, bytes, hipMemcpyDeviceToDevice, b->stream)
-                               : hipMemcpyAsync(keep_a, d0, bytes, hipMemcpyDeviceToDevice, b->stream);
-        if (r == hipSuccess && b->fp32c())
-            r = restore ? hipMemcpyAsync(b->qlo[start], keep_b, bytes, hipMemcpyDeviceToDevice, b->stream)
-                        : hipMemcpyAsync(keep_b, b->qlo[start], bytes, hipMemcpyDeviceToDevice, b->stream);
+        hipError_t r = copy(d0, keep_a, bytes);
+        if (r == hipSuccess && b->fp32c()) r = copy(b->qlo[start], keep_b, bytes);
+        if (r == hipSuccess) r = copy(b->d_time, keep_t, tbytes);
+        if (r == hipSuccess) r = copy(b->d_dt, keep_t + b->nprob, tbytes);
         if (r == hipSuccess)
-            r = restore ? hipMemcpyAsync(b->d_time, keep_t, b->nprob * sizeof(double), hipMemcpyDeviceToDevice, b->stream)
-                        : hipMemcpyAsync(keep_t, b->d_time, b->nprob * sizeof(double), hipMemcpyDeviceToDevice, b->stream);
+            r = restore ? hipMemcpyAsync(b->floor_hits, &keep_hits, sizeof(keep_hits), hipMemcpyHostToDevice, b->stream)
+                        : hipMemcpyAsync(&keep_hits, b->floor_hits, sizeof(keep_hits), hipMemcpyDeviceToHost, b->stream);
+        if (r == hipSuccess) r = hipStreamSynchronize(b->stream);  // keep_hits is a host word
         return r;
     };
     if (e == hipSuccess) e = stash(false);

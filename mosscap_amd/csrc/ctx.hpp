@@ -68,6 +68,11 @@ struct mh_ctx {
     int cur = 0;         // buffer holding the current state
     int qn = 0;          // buffer holding Q^n during a step
     int next_stage = 0;  // mh_stage bookkeeping
+    // The clock's CFL slots hold the largest signal speed of buffer `cur`
+    // (set by mh_run, whose last stage folds it; cleared by every call that
+    // changes the state or the slots otherwise), so the next mh_run needs no
+    // separate CFL pass.
+    bool amax_valid = false;
     int span_h = 16;
     int strips = 0;
     std::string err;

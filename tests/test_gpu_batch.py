@@ -95,15 +95,26 @@ def test_batch_fp32c_close_to_fp64():
     assert num / den < 1e-5, num / den
 
 
-def test_batch_autotune_restores_state():
-    lengths, q0 = _small(nprob=32, seed=5, n_max=600)
-    with BatchContext(lengths) as a, BatchContext(lengths) as b:
-        a.upload(q0)
-        a.fill_bcs()
-        b.upload(q0)
-        b.fill_bcs()
+@pytest.mark.parametrize("form", [M.FP64, M.FP32C])
+def test_batch_autotune_restores_state(form):
+    """The trial runs of the autotuner leave no trace: state, per-problem time and last dt, and the
+    floor-hit count are what they were before it (FP32C: seed 11 holds near-vacuum members, and
+    p_floor 1e-2 sits at the bottom of the ensemble's pressure range, so the floors fire)."""
+    lengths, q0 = _small(nprob=32, seed=11 if form == M.FP32C else 5, n_max=600)
+    kw = dict(rho_floor=1e-10, p_floor=1e-2) if form == M.FP32C else {}
+    with BatchContext(lengths, precision_form=form, **kw) as a, BatchContext(lengths, precision_form=form, **kw) as b:
+        for c in (a, b):
+            c.upload(q0)
+            c.fill_bcs()
+            c.run(2)
         best, table = a.autotune(steps=2, waves_per_cu=[1, 4])
         assert len(table) == 2
+        for x, y in zip(a.times(), b.times()):
+            np.testing.assert_array_equal(x, y)
+        assert a.stats().floor_hits == b.stats().floor_hits
+        if form == M.FP32C:
+            assert a.stats().floor_hits > 0
+        np.testing.assert_array_equal(a.download(), b.download())
         a.run(5)
         b.run(5)
         np.testing.assert_array_equal(a.download(), b.download())

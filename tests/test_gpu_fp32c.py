@@ -86,18 +86,31 @@ def test_sod_256_fp32c():
     assert r[2] == 0.0, r
 
 
-def test_floors_count_and_hold():
-    """Sedov-type blast at small size with floors on: no NaN, hits counted."""
+@pytest.mark.parametrize("p_floor,hits", [(1e-9, "none"), (1e-4, "some")])
+def test_floors_count_and_hold(p_floor, hits):
+    """Sedov blast at 128^2 (ambient p 1e-5), 60 steps with floors on. Below the ambient pressure
+    the floors never fire (FP64 pressure recovery keeps p positive): zero hits. Above it they fire
+    on every ambient cell of every stage: at least 3 x 60 x (ambient cells) / 2 hits. Either way the
+    state stays finite with positive density and pressure."""
     cfg = problems.sedov_config(128)
     q0 = problems.sedov(cfg)
-    with HydroContext(cfg, precision_form=M.FP32C, rho_floor=1e-8, p_floor=1e-9) as ctx:
+    g = cfg.ng
+    with HydroContext(cfg, precision_form=M.FP32C, rho_floor=1e-8, p_floor=p_floor) as ctx:
         ctx.upload(q0)
         ctx.fill_bcs()
         ctx.run(60)
         q = ctx.download()
         st = ctx.stats()
     assert np.isfinite(q).all()
-    assert st.floor_hits >= 0
+    inner = q[:, 0, g:-g, g:-g]
+    p = 0.4 * (inner[3] - 0.5 * (inner[1] ** 2 + inner[2] ** 2) / inner[0])
+    assert inner[0].min() > 0.0 and p.min() > 0.0
+    if hits == "none":
+        assert st.floor_hits == 0
+    else:
+        ambient = int((p < 1e-4).sum())
+        assert ambient > cfg.nx * cfg.ny // 2
+        assert st.floor_hits >= 3 * 60 * ambient // 2, (st.floor_hits, ambient)
 
 
 def test_fp32_functors_close_to_fp64(orc):
@@ -120,14 +133,29 @@ def test_fp32_functors_close_to_fp64(orc):
         np.testing.assert_allclose(f, fo, rtol=1e-4, atol=1e-5)
 
 
-@pytest.mark.parametrize("bc", ["periodic", "zerograd"])
-def test_ghosts_read_from_sources_equal_filled_ghosts(bc):
+NSTAGES = {"rk2": 2, "ssprk3": 3, "ssprk4": 4}
+
+
+@pytest.mark.parametrize("bcx,bcy,recon,stepper", [
+    ("periodic", "periodic", "weno5z", "ssprk3"),
+    ("zerograd", "zerograd", "weno5z", "ssprk3"),
+    ("periodic", "zerograd", "weno5z", "ssprk3"),
+    ("zerograd", "periodic", "weno5z", "ssprk3"),
+    ("periodic", "zerograd", "weno5z", "rk2"),
+    ("zerograd", "periodic", "weno5z", "ssprk4"),
+    ("periodic", "zerograd", "muscl", "ssprk4"),
+    ("zerograd", "periodic", "ppm", "rk2"),
+])
+def test_ghosts_read_from_sources_equal_filled_ghosts(bcx, bcy, recon, stepper):
     """On grids whose faces are all periodic or zerograd the compensated form reads ghost rows and
     columns from their source cells and fills ghosts once per step. A user_bc hook that does
     nothing switches that off (every stage fills its ghosts); both runs, the graph-replayed loop,
-    the loop without graphs, mh_time_step and mh_stage must give the same bits."""
+    the loop without graphs, mh_time_step and mh_stage must give the same bits. Mixed faces (x
+    periodic, y zerograd and the transpose) catch a swapped axis; RK2 and SSPRK(4,3) rotate the
+    buffers differently from SSPRK3."""
     cfg = problems.kh_config(128)
-    cfg.bc = (bc,) * 4
+    cfg.bc = (bcx, bcx, bcy, bcy)
+    cfg.reconstruction, cfg.time_stepper = recon, stepper
     cfg.nx, cfg.ny = 150, 70  # strips and spans that do not divide the grid
     q0 = problems.kh_band(cfg)
     out = {}
@@ -146,7 +174,7 @@ def test_ghosts_read_from_sources_equal_filled_ghosts(bc):
                     if mode == "steps":
                         ctx.time_step(dt)
                     else:
-                        for s in range(3):
+                        for s in range(NSTAGES[stepper]):
                             ctx.stage(s, dt)
                     t += dt
             out[mode] = ctx.download()
@@ -154,12 +182,14 @@ def test_ghosts_read_from_sources_equal_filled_ghosts(bc):
         np.testing.assert_array_equal(out[mode], out["graph"], err_msg=mode)
 
 
-@pytest.mark.parametrize("bc", ["periodic", "zerograd"])
-def test_ghosts_read_from_sources_3d(bc):
+@pytest.mark.parametrize("bcs", [("periodic",) * 6, ("zerograd",) * 6,
+                                 ("periodic", "periodic", "zerograd", "zerograd", "periodic", "periodic"),
+                                 ("zerograd", "zerograd", "periodic", "periodic", "zerograd", "zerograd")])
+def test_ghosts_read_from_sources_3d(bcs):
     """3-D: the z sweep reads ghost planes and the x-y march ghost rows and columns from their
     sources; graph-replayed steps must equal steps driven stage by stage (mh_stage fills every
     stage's ghosts) bit for bit."""
-    cfg = Config(num_dim=3, nx=70, ny=40, nz=20, ng=3, bc=(bc,) * 6, reconstruction="weno5z",
+    cfg = Config(num_dim=3, nx=70, ny=40, nz=20, ng=3, bc=bcs, reconstruction="weno5z",
                  riemann_solver="hllc", time_stepper="ssprk3", gamma=1.4, max_cfl=0.3)
     q0 = problems.blast_3d(cfg, seed=5, p_ratio=3.0)
     out = {}

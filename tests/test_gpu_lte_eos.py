@@ -74,6 +74,40 @@ def test_fields_follow_the_last_stage_input():
     np.testing.assert_array_equal(ya, yb)
 
 
+@pytest.mark.parametrize("bc", ["periodic", "zerograd"])
+@pytest.mark.parametrize("entry", ["time_step", "run", "autotune"])
+def test_fields_of_last_stage_input_with_source_read_ghosts(bc, entry):
+    """FP32C on periodic / zerograd grids reads ghosts from their source cells, so a step leaves the
+    ghosts of its intermediate stage inputs unfilled; eos_fields(which=1) must still see that input
+    as mh_stage (which fills every stage's ghosts) leaves it, ghosts included. After the autotuner
+    the fields come from the current state."""
+    cfg = _cfg()
+    cfg.bc = (bc,) * 4
+    q0 = _solar_like(cfg)
+    with M.HydroContext(cfg, precision_form=M.FP32C) as a, M.HydroContext(cfg, precision_form=M.FP32C) as b:
+        for c in (a, b):
+            c.upload(q0)
+            c.fill_bcs()
+        dt = a.compute_dt(0.0, 0.0)
+        if entry == "time_step":
+            a.time_step(dt)
+        elif entry == "run":
+            a.run(1)
+        else:
+            a.time_step(dt)
+            a.autotune(steps=1, candidates=[8, 16])
+        if entry == "autotune":
+            b.time_step(dt)
+            b.fill_bcs()
+        else:
+            b.stage(0, dt)
+            b.stage(1, dt)  # b's current state is the input of a's last stage
+        ya, Ta = a.eos_fields(which=1)
+        yb, Tb = b.eos_fields(which=0)
+    np.testing.assert_array_equal(Ta, Tb)
+    np.testing.assert_array_equal(ya, yb)
+
+
 def test_host_driver_reports_inverted_bracket(tmp_path):
     """kh_band's dimensionless state read as SI units is a gas at ~1e-3 K, below the EOS's 100 K
     floor: the reference aborts ("Temperature bounds flipped!"); the host driver raises it."""
