@@ -65,12 +65,17 @@ typedef struct {
 typedef struct {
     int32_t precision_form; /* MH_FORM_* */
     int32_t span_rows;      /* rows marched per wavefront in 2-D; 0 = tuned default */
-    float rho_floor;        /* FP32C positivity floors on the reconstruction input; <= 0 disables */
+    float rho_floor;        /* FP32C positivity floors on the reconstruction input (cells and
+                             * faces) and on each stage's new state (a cell below rho_floor is
+                             * put at rest at (rho_floor, p_floor); one below p_floor gets its
+                             * energy raised to p_floor); <= 0 disables. No counterpart in the
+                             * reference (whose plain method has no floors). */
     float p_floor;
 } mh_options;
 
 typedef struct {
-    uint64_t floor_hits; /* FP32C: (interior cell, stage) pairs whose rho or p was floored */
+    uint64_t floor_hits; /* FP32C: (interior cell, stage) pairs whose input or new state took a
+                          * floor (each pair counted once) */
     int64_t steps;       /* steps taken by this context */
     double last_dt;      /* dt of the last step */
     double time;         /* device clock after mh_run */
@@ -256,6 +261,16 @@ int mh_device_state(mh_ctx* ctx, mh_device_view* out);
 int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double max_cfl, const double* dts_in,
                        double* dts_out);
 int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_initial);
+/* Tangent-linear run of the same recorded steps (forward mode, no
+ * counterpart in the reference): from the first checkpoint and a direction
+ * dq0 (host array laid out like Q, ghost entries ignored) it evaluates every
+ * stage on dual numbers, with the drop-in's expressions and the recorded dt,
+ * and returns dQ^N/dQ^0 . dq0 in dq_final (ghost-filled); q_final (may be
+ * NULL) receives the value part, Q^N, bit for bit the forward result. The
+ * exact directional derivative that mh_adjoint_backward's gradient must
+ * match: <lambda_initial, dq0> = <lambda_final, dq_final>. Constant boundaries
+ * are refused. Unoptimised (one thread per cell reloads its stencil). */
+int mh_adjoint_tangent(mh_ctx* ctx, const double* dq0, double* dq_final, double* q_final);
 /* Device time (HIP events on the context's stream) of the last
  * mh_adjoint_backward: segment recomputation plus reverse stages, the two
  * host transfers excluded. */

@@ -126,6 +126,7 @@ _SIGNATURES = [
     ("mh_adjoint_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward_ms", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("mh_adjoint_tangent", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_probe_vjp", C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     ("mh_batch_create", C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.POINTER(MhBoundaries),
                                   C.POINTER(MhSchemes), C.c_double, C.POINTER(MhOptions), C.POINTER(C.c_void_p)]),
@@ -395,6 +396,14 @@ class HydroContext:
         out = np.empty(self.cfg.shape)
         self._c(lib().mh_adjoint_backward(self._h, _ptr(lf), _ptr(out)))
         return out
+
+    def adjoint_tangent(self, dq0: np.ndarray):
+        """Forward-mode derivative of the recorded run along dq0 (shaped like Q; ghost entries
+        ignored): returns (dQ^N, Q^N), both ghost-filled; Q^N repeats the forward result."""
+        d = np.ascontiguousarray(dq0, dtype=np.float64).reshape(self.cfg.shape)
+        dq, q = np.empty(self.cfg.shape), np.empty(self.cfg.shape)
+        self._c(lib().mh_adjoint_tangent(self._h, _ptr(d), _ptr(dq), _ptr(q)))
+        return dq, q
 
     def adjoint_backward_ms(self) -> float:
         """Device milliseconds of the last adjoint_backward (recompute + reverse, transfers excluded)."""

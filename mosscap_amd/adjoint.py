@@ -64,6 +64,20 @@ class KHAdjointProblem:
         return l0[0] + self.vx * l0[1] + self.vy * l0[2] + 0.5 * (self.vx ** 2 + self.vy ** 2) * l0[3]
 
 
+    def tangent(self, direction: np.ndarray):
+        """Exact directional derivative of the final interior density along a control direction
+        (forward mode over the recorded, frozen-dt steps), with the final density itself."""
+        g = self.cfg.ng
+        dq = np.zeros(self.cfg.shape)
+        d = np.asarray(direction, dtype=np.float64)
+        dq[0, 0, g:-g, g:-g] = d
+        dq[1, 0, g:-g, g:-g] = self.vx * d
+        dq[2, 0, g:-g, g:-g] = self.vy * d
+        dq[3, 0, g:-g, g:-g] = 0.5 * (self.vx ** 2 + self.vy ** 2) * d
+        dqn, qn = self.ctx.adjoint_tangent(dq)
+        return dqn[0, 0, g:-g, g:-g].copy(), qn[0, 0, g:-g, g:-g].copy()
+
+
 def kh_adjoint_function():
     """Build the torch.autograd.Function (imports torch lazily)."""
     import torch

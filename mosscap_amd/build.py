@@ -56,6 +56,7 @@ UNITS = [
     ("kernels/batch_f64.hip", "off"),
     ("kernels/batch_f32c.hip", "fast"),
     ("kernels/adjoint.hip", "off"),
+    ("kernels/tangent.hip", "off"),
     ("capi.cpp", "off"),
     ("capi_batch.cpp", "off"),
     ("capi_adjoint.cpp", "off"),

@@ -172,6 +172,71 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_adjoint_backward");
 }
 
+int mh_adjoint_tangent(mh_ctx* ctx, const double* dq0, double* dq_final, double* q_final) {
+    if (!ctx || !dq0 || !dq_final) return MH_ERR_INVALID;
+    const int64_t nsteps = (int64_t)ctx->adj_dts.size();
+    if (nsteps == 0 || ctx->ckpt.empty()) return ctx->fail(MH_ERR_INVALID, "run mh_adjoint_forward first");
+    const int32_t bcs[4] = {ctx->bdry.xs, ctx->bdry.xe, ctx->bdry.ys, ctx->bdry.ye};
+    for (int32_t b : bcs)
+        if (b == (int32_t)BoundaryType::Constant)  // its ghosts are fixed values, their derivative zero
+            return ctx->fail(MH_ERR_UNSUPPORTED, "no tangent for constant boundaries");
+    hipError_t e = hipSetDevice(ctx->device);
+    const size_t nv = (size_t)ctx->nvals;
+    // values and derivatives of three rotating states
+    DevVec val[3], der[3];
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        e = val[k].alloc(nv);
+        if (e == hipSuccess) e = der[k].alloc(nv);
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_tangent: scratch");
+    // the remaining boundary types fill ghosts by copies and sign changes of
+    // interior cells, so the same fill maps derivatives to ghost derivatives
+    const auto fill = [&](double* q) {
+        return launch_ghost_f64(2, ctx->ghost_grid(0), FormF64::Planes{q}, FormF64::Planes{q}, nullptr, ctx->stream);
+    };
+    e = hipMemcpyAsync(val[0].p, ctx->ckpt[0], nv * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(der[0].p, dq0, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = fill(der[0].p);
+    TanStage a{};
+    a.xc = ctx->sz.xc;
+    a.yc = ctx->sz.yc;
+    a.ng = ctx->sz.ng;
+    a.nx = a.xc - 2 * a.ng;
+    a.ny = a.yc - 2 * a.ng;
+    a.gamma = ctx->gamma;
+    a.dx = ctx->dx;
+    a.gravity = ctx->gravity;
+    a.gx = ctx->grav[0];
+    a.gy = ctx->grav[1];
+    const int ns = num_stages(ctx->sch.stepper);
+    int k = 0;  // buffer of Q^n
+    for (int64_t n = 0; n < nsteps && e == hipSuccess; ++n) {
+        a.dt = ctx->adj_dts[n];
+        int in = k;
+        for (int s = 0; s < ns && e == hipSuccess; ++s) {
+            const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
+            a.qv = val[in].p;
+            a.qd = der[in].p;
+            a.nv = val[k].p;
+            a.nd = der[k].p;
+            a.ov = val[out].p;
+            a.od = der[out].p;
+            a.kind = stage_kind(ctx->sch.stepper, s);
+            e = launch_tangent_stage(ctx->sch, a, ctx->stream);
+            if (e == hipSuccess) e = fill(val[out].p);
+            if (e == hipSuccess) e = fill(der[out].p);
+            in = out;
+        }
+        k = in;
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dq_final, der[k].p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && q_final)
+        e = hipMemcpyAsync(q_final, val[k].p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? MH_OK : ctx->hip_fail(e, "mh_adjoint_tangent");
+}
+
 int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms) {
     if (!ctx || !ms) return MH_ERR_INVALID;
     *ms = ctx->adj_bwd_ms;

@@ -12,8 +12,9 @@
 //   to the hi+lo Q^n with TwoSum, so the state accumulates the small
 //   per-stage increments without FP32 rounding drift.
 // * The CFL signal speed is evaluated in FP64 from hi+lo.
-// * Optional positivity floors on rho and p of the reconstruction input; each
-//   interior cell and stage that needed one is counted once.
+// * Optional positivity floors on rho and p of the reconstruction input and
+//   of each stage's new state; each interior cell and stage that needed one
+//   is counted once.
 #pragma once
 
 #include "functors.hpp"
@@ -202,6 +203,43 @@ struct FormF32c {
             case 4: return df::add_f(Q, 0.5f * dq);             // Q + dq / 2
             default: return around_qn(1.0f / 3.0f, 0.5f);       // 2/3 Q^n + 1/3 Q + 1/6 dq
         }
+    }
+
+    // Positivity floors of a stage's new state (with the floors on): a
+    // density below rho_floor (or NaN) puts the cell at rest at rho_floor and
+    // p_floor; otherwise a pressure below p_floor (or NaN) raises the energy
+    // to the kinetic energy plus p_floor/(gamma-1). Floors on the
+    // reconstruction input alone cannot stop a near-vacuum cell from being
+    // emptied below zero by its outflow, after which 1/rho of the floored
+    // density turns its momentum into an unbounded velocity. Returns true
+    // when it changed the cell.
+    template <int ND, int NV>
+    __device__ static inline bool floor_state(const Ptrs& p, float2 (&q)[NV]) {
+        if (!(p.rho_floor > 0.0f)) return false;
+        const double rho = df::to_double(q[0]);
+        const double pf = (double)p.p_floor;
+        if (!(rho >= (double)p.rho_floor)) {
+            q[0] = df::from_double((double)p.rho_floor);
+#pragma unroll
+            for (int a = 0; a < ND; ++a) q[1 + a] = make_float2(0.0f, 0.0f);
+            q[1 + ND] = df::from_double(pf / p.gm1);
+            return true;
+        }
+        const double r0 = (double)__builtin_amdgcn_rcpf((float)rho);
+        const double r = (rho < 1e30) ? r0 * (2.0 - rho * r0) : 1.0 / rho;
+        double m2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) {
+            const double m = df::to_double(q[1 + a]);
+            m2 += m * m;
+        }
+        const double ke = 0.5 * m2 * r;
+        const double pres = p.gm1 * (df::to_double(q[1 + ND]) - ke);
+        if (!(pres >= pf)) {
+            q[1 + ND] = df::from_double(ke + pf / p.gm1);
+            return true;
+        }
+        return false;
     }
 
     // Reconstructed face states get the same floors as the cell states

@@ -84,37 +84,42 @@ struct FormF64 {
     // no floors in the drop-in (the reference has none)
     template <int ND, int NV>
     __device__ static inline void floor_face(const Ptrs&, double (&)[NV]) {}
+    template <int ND, int NV>
+    __device__ static inline bool floor_state(const Ptrs&, double (&)[NV]) { return false; }
 
     // TimeStepping.cpp stage lambdas (RK2 60-67/80-87, SSPRK3 137-144/157-164/
-    // 177-184, SSPRK(4,3) 234-301), grouped exactly as written there.
-    __device__ static inline double combine(int kind, double Q, double Qn, double dq, double src) {
+    // 177-184, SSPRK(4,3) 234-301), grouped exactly as written there. Generic
+    // over the real type so that the tangent-linear stage (dual numbers)
+    // evaluates the same expressions; T = double is the drop-in's.
+    template <class T>
+    __device__ static inline T combine(int kind, T Q, T Qn, T dq, T src) {
         switch (kind) {
             case 0: return Q + (dq + src);
-            case 1: return 0.5 * (Qn + Q + dq + src);
-            case 2: return 0.75 * Qn + 0.25 * (Q + dq + src);
-            case 3: return (1.0 / 3.0) * Qn + (2.0 / 3.0) * (Q + dq + src);
-            case 4: return Q + 0.5 * (dq + src);
-            default: return (2.0 / 3.0) * Qn + (1.0 / 3.0) * Q + (1.0 / 6.0) * (dq + src);
+            case 1: return T(0.5) * (Qn + Q + dq + src);
+            case 2: return T(0.75) * Qn + T(0.25) * (Q + dq + src);
+            case 3: return T(1.0 / 3.0) * Qn + T(2.0 / 3.0) * (Q + dq + src);
+            case 4: return Q + T(0.5) * (dq + src);
+            default: return T(2.0 / 3.0) * Qn + T(1.0 / 3.0) * Q + T(1.0 / 6.0) * (dq + src);
         }
     }
 
     // Source S*dt of the stage lambdas for a registered gravity term:
     // gravity_kernel (Gravity.cpp:22-38) accumulated into a zeroed S from the
-    // stage input Q of the cell, then multiplied by dt.
-    template <int ND, int NV>
-    __device__ static inline void gravity_src(const Ptrs& p, const double (&q)[NV], double dt, double (&src)[NV]) {
-        src[0] = 0.0 * dt;
-        src[1] = (0.0 + q[0] * p.gx) * dt;
-        double energy_update = q[1] * p.gx;
+    // stage input Q of the cell, then multiplied by dt (generic like combine).
+    template <int ND, int NV, class T>
+    __device__ static inline void gravity_src(const Ptrs& p, const T (&q)[NV], T dt, T (&src)[NV]) {
+        src[0] = T(0.0) * dt;
+        src[1] = (T(0.0) + q[0] * T(p.gx)) * dt;
+        T energy_update = q[1] * T(p.gx);
         if constexpr (ND > 1) {
-            src[2] = (0.0 + q[0] * p.gy) * dt;
-            energy_update += q[2] * p.gy;
+            src[2] = (T(0.0) + q[0] * T(p.gy)) * dt;
+            energy_update += q[2] * T(p.gy);
         }
         if constexpr (ND > 2) {
-            src[3] = (0.0 + q[0] * p.gz) * dt;
-            energy_update += q[3] * p.gz;
+            src[3] = (T(0.0) + q[0] * T(p.gz)) * dt;
+            energy_update += q[3] * T(p.gz);
         }
-        src[1 + ND] = (0.0 + energy_update) * dt;
+        src[1 + ND] = (T(0.0) + energy_update) * dt;
     }
 
     // dt_reducer, Hydro.cpp:192-206: cons_to_prim then c_s + |v|.

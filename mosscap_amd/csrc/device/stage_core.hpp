@@ -159,6 +159,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         }
     };
     unsigned int floor_hits = 0;
+    // bit k: a floor fired in the cons->prim of the row committed k rows ago
+    // (the row combined next was committed ORD + 1 rows before it)
+    uint32_t in_floor_bits = 0;
 
     auto commit_row = [&](int r, const Raw (&q)[NV]) {
         const int slot = r & (RING - 1);
@@ -197,8 +200,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         }
         T w[NV];
         const bool hit = Form::template to_prim_nd<ND>(p, q, w);
-        // each interior cell is committed by exactly one (strip, span) owner
-        floor_hits += (hit && out_lane && r >= j0 && r < j1) ? 1u : 0u;
+        in_floor_bits = (in_floor_bits << 1) | (hit ? 1u : 0u);
 #pragma unroll
         for (int v = 0; v < NV; ++v) ring[slot][v][lane + ORD] = w[v];
         if constexpr (NH > 0) {
@@ -336,6 +338,12 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 if constexpr (ND > 2) q_update += p.dz3[v * plane + pk + j * xc + c];
                 q_update *= dtdx;
                 qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v]);
+            }
+            // each interior cell is combined by exactly one (strip, span) owner
+            const bool st_hit = Form::template floor_state<ND>(p, qo);
+            floor_hits += (st_hit || ((in_floor_bits >> (ORD + 1)) & 1u)) ? 1u : 0u;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
                 if constexpr (kBuf) Form::bput(bout, (uint32_t)c * ES, (uint32_t)(v * plane + row), qo[v]);
                 else Form::put(p.out, v * plane + row + c, qo[v]);
             }
@@ -397,7 +405,7 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
     }
     T w[NV];
     const bool out_lane = lane >= 1 && lane <= kStripOut && c < ng + nx;
-    floor_hits += (Form::template to_prim_nd<1>(p, q, w) && out_lane) ? 1u : 0u;
+    const bool in_hit = Form::template to_prim_nd<1>(p, q, w);
     wave_lds_sync();
 #pragma unroll
     for (int v = 0; v < NV; ++v) lds_row[v * ROW + lane + ORD] = w[v];
@@ -436,8 +444,11 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
             q_update += Fx[v] - Fxp[v];
             q_update *= dtdx;
             qo[v] = Form::combine(kind, q[v], qn[v], q_update, srcv[v]);
-            Form::put(p.out, base + v * xc + c, qo[v]);
         }
+        const bool st_hit = Form::template floor_state<1>(p, qo);
+        floor_hits += (in_hit || st_hit) ? 1u : 0u;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Form::put(p.out, base + v * xc + c, qo[v]);
         if (reduce_dt) {
             const double a = Form::template signal_speed_nd<1>((double)p.gamma, qo);
             amax = a > amax ? a : amax;

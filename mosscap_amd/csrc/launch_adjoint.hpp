@@ -31,6 +31,23 @@ struct AdjStage {
 };
 
 hipError_t launch_adjoint_stage(const Schemes& sch, const AdjGrid& g, const AdjStage& st, hipStream_t stream);
+
+// One tangent-linear stage (kernels/tangent.hip): values and derivatives of
+// Q^(s) and Q^n (ghost-filled) to those of Q^(s+1) on the interior cells.
+struct TanStage {
+    const double* qv;  // Q^(s) values and derivatives
+    const double* qd;
+    const double* nv;  // Q^n
+    const double* nd;
+    double* ov;        // Q^(s+1)
+    double* od;
+    int32_t xc, yc, ng, nx, ny;
+    int32_t kind;      // StageKind
+    double gamma, dt, dx;
+    int32_t gravity;
+    double gx, gy;
+};
+hipError_t launch_tangent_stage(const Schemes& sch, const TanStage& a, hipStream_t st);
 // y = a*x + b*y on the interior, 0 on ghosts.
 hipError_t launch_adjoint_axpby(const AdjGrid& g, double a, const double* x, double b, double* y, hipStream_t st);
 // unit probes of the hand VJPs against dual-number Jacobians

@@ -201,5 +201,7 @@ def test_batch_4096_problems_500_steps_fp32c():
         ref = out[("graph", k)]
         got = out[(mode, k)]
         np.testing.assert_array_equal(got[0], ref[0], err_msg=f"{mode} {k}")
-        np.testing.assert_allclose(got[1], ref[1], rtol=1e-15, atol=0, err_msg=f"{mode} {k}")
+        # the clocks accumulate the FP64 dt, whose last bits may differ between the two modes'
+        # CFL kernels; the stages use dt rounded to FP32, so the states above agree bit for bit
+        np.testing.assert_allclose(got[1], ref[1], rtol=1e-13, atol=0, err_msg=f"{mode} {k}")
         assert got[2] == ref[2], (mode, k)
