@@ -275,6 +275,28 @@ int mh_adjoint_tangent(mh_ctx* ctx, const double* dq0, double* dq_final, double*
  * mh_adjoint_backward: segment recomputation plus reverse stages, the two
  * host transfers excluded. */
 int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms);
+/* ---- one step without device allocations (no counterpart in the reference) --
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. The step of
+ * mh_compute_dt + mh_time_step (FP64 drop-in arithmetic, bit for bit the same
+ * result) on a caller-owned state, with no context: q_in is Q^n on the device
+ * ([var][k][j][i] FP64, ghosts filled), q_out receives Q^{n+1} (a distinct
+ * device buffer of the same size; ghosts filled). dt > 0 takes that step;
+ * dt <= 0 takes the CFL step max_cfl * dx / max(c_s + |v|) of q_in, ghosts
+ * included (Hydro.cpp:208-229, no output-time clamp). *dt_used (may be NULL)
+ * receives it. 1-D and 2-D grids, no tracers, gravity or userfn faces.
+ * The step clock and the intermediate stage states live in a fixed device
+ * array of MH_NOALLOC_SCRATCH_BYTES per device, held until the process ends:
+ * grids needing more than 2 states of (bytes - 16.6 KB) / 16 values each
+ * (about 1 M cells of 4 variables) are refused with MH_ERR_INVALID. Calls
+ * that overlap on one device (other threads, other streams) race on that
+ * array. Runs on `stream` (a hipStream_t; NULL = the null stream) and is
+ * synchronous on return. Errors: mh_last_error(NULL). */
+#define MH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mh_step_noalloc(const mh_grid_size* sz, double dx, const mh_boundaries* bdry, int32_t num_dim,
+                    const mh_schemes* schemes, double gamma, double max_cfl, double dt, const double* q_in,
+                    double* q_out, double* dt_used, void* stream);
+
 /* probe of the hand-written VJPs against dual-number Jacobians (what: 0/1 WENO5-Z
  * hand/dual, in[n][5], cot[n][2] -> out[n][5]; 2/3 Rusanov x hand/dual, 4/5 Rusanov y,
  * 6/7 HLLC x, 8/9 HLLC y: in[n][8] (wL, wR), cot[n][4] -> out[n][8]). */

@@ -127,6 +127,8 @@ _SIGNATURES = [
     ("mh_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_adjoint_backward_ms", C.c_int, [C.c_void_p, C.c_void_p]),
     ("mh_adjoint_tangent", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("mh_step_noalloc", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int32, C.c_void_p, C.c_double,
+                                  C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("mh_probe_vjp", C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     ("mh_batch_create", C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.POINTER(MhBoundaries),
                                   C.POINTER(MhSchemes), C.c_double, C.POINTER(MhOptions), C.POINTER(C.c_void_p)]),
@@ -228,6 +230,34 @@ class Config:
         return (self.nvar, self.zc, self.yc, self.xc)
 
 
+def _abi_structs(cfg: Config):
+    """The grid size, boundaries and schemes of `cfg` as C-ABI structs."""
+    sz = MhGridSize(cfg.xc, cfg.yc, cfg.zc, cfg.ng)
+    b = MhBoundaries()
+    names = ("xs", "xe", "ys", "ye", "zs", "ze")
+    for k, (n, kind) in enumerate(zip(names, cfg.bc)):
+        setattr(b, n, BOUNDARY[kind.lower()])
+        arr = getattr(b, n + "_const")
+        for i, v in enumerate(cfg.bc_const[k] if k < len(cfg.bc_const) else ()):
+            arr[i] = v
+    sch = MhSchemes(RECONSTRUCTION[cfg.reconstruction], SLOPE_LIMITER[cfg.slope_limiter],
+                    RIEMANN_SOLVER[cfg.riemann_solver], TIME_STEPPER[cfg.time_stepper])
+    return sz, b, sch
+
+
+def step_noalloc(cfg: Config, q_in_dev: int, q_out_dev: int, dt: float = 0.0, max_cfl: float | None = None,
+                 stream: int = 0) -> float:
+    """One FP64 step of a caller-owned device state without device allocations (mh_step_noalloc):
+    q_in_dev / q_out_dev are device addresses of ghost-filled [var][k][j][i] float64 arrays. Returns
+    the dt used (the CFL dt of q_in when dt <= 0)."""
+    sz, b, sch = _abi_structs(cfg)
+    used = C.c_double()
+    _check(lib().mh_step_noalloc(C.byref(sz), cfg.dx, C.byref(b), cfg.num_dim, C.byref(sch), cfg.gamma,
+                                 cfg.max_cfl if max_cfl is None else max_cfl, dt, C.c_void_p(q_in_dev),
+                                 C.c_void_p(q_out_dev), C.byref(used), C.c_void_p(stream or None)))
+    return used.value
+
+
 class HydroContext:
     """One problem on the GPU: owns the C-ABI context."""
 
@@ -235,17 +265,8 @@ class HydroContext:
                  rho_floor: float = 0.0, p_floor: float = 0.0):
         L = lib()
         self.cfg = cfg
-        sz = MhGridSize(cfg.xc, cfg.yc, cfg.zc, cfg.ng)
+        sz, b, sch = _abi_structs(cfg)
         loc = MhGridLoc(cfg.x_start, cfg.y_start, cfg.z_start)
-        b = MhBoundaries()
-        names = ("xs", "xe", "ys", "ye", "zs", "ze")
-        for k, (n, kind) in enumerate(zip(names, cfg.bc)):
-            setattr(b, n, BOUNDARY[kind.lower()])
-            arr = getattr(b, n + "_const")
-            for i, v in enumerate(cfg.bc_const[k] if k < len(cfg.bc_const) else ()):
-                arr[i] = v
-        sch = MhSchemes(RECONSTRUCTION[cfg.reconstruction], SLOPE_LIMITER[cfg.slope_limiter],
-                        RIEMANN_SOLVER[cfg.riemann_solver], TIME_STEPPER[cfg.time_stepper])
         opt = MhOptions(precision_form, span_rows, rho_floor, p_floor)
         h = C.c_void_p()
         _check(L.mh_create(C.byref(sz), cfg.dx, C.byref(loc), C.byref(b), cfg.num_dim, cfg.num_tracers, C.byref(sch),

@@ -60,6 +60,7 @@ UNITS = [
     ("capi.cpp", "off"),
     ("capi_batch.cpp", "off"),
     ("capi_adjoint.cpp", "off"),
+    ("capi_noalloc.cpp", "off"),
 ]
 
 

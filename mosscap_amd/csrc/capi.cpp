@@ -13,6 +13,8 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+void mh::set_last_error(const std::string& msg) { g_create_error = msg; }
+
 extern "C" {
 
 const char* mh_last_error(const mh_ctx* ctx) {
@@ -103,21 +105,7 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
     c->ncells = ncells;
     c->nvals = ncells * nvar;
     c->strips = (nx + kStripOut - 1) / kStripOut;
-    if (c->opt.span_rows > 0) {
-        c->span_h = c->opt.span_rows;
-    } else {
-        // One round of wavefronts: the shortest span (>= 8 rows; the y halo
-        // costs 2R+2 rows per span) whose wavefronts all fit 1024 SIMDs at 4
-        // waves each, so none waits for a second round; 32 rows when even
-        // that needs several rounds.
-        c->span_h = 32;
-        for (int h = 8; h <= 32; ++h) {
-            if ((int64_t)c->strips * ((ny + h - 1) / h) * nz <= 4096) {
-                c->span_h = h;
-                break;
-            }
-        }
-    }
+    c->span_h = c->opt.span_rows > 0 ? c->opt.span_rows : default_span_rows(c->strips, ny, nz);
 
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -21,6 +21,19 @@ inline int stage_kind(TimeStepScheme ts, int stage) {
 
 inline bool valid_bc(int32_t b) { return b >= 0 && b <= 6; }
 
+// Default rows per wavefront of the 2-D march: one round of wavefronts, the
+// shortest span (>= 8 rows; the y halo costs 2R+2 rows per span) whose
+// wavefronts all fit 1024 SIMDs at 4 waves each, so none waits for a second
+// round; 32 rows when even that needs several rounds.
+inline int default_span_rows(int strips, int ny, int nz) {
+    for (int h = 8; h <= 32; ++h)
+        if ((int64_t)strips * ((ny + h - 1) / h) * nz <= 4096) return h;
+    return 32;
+}
+
+// Message returned by mh_last_error(NULL) (functions without a context).
+void set_last_error(const std::string& msg);
+
 }  // namespace mh
 
 using namespace mh;
