@@ -257,7 +257,9 @@ int mh_device_state(mh_ctx* ctx, mh_device_view* out);
  * Q, ghost entries ignored) and returns dJ/dQ^0 (lambda_initial, host, ghost
  * entries zero), recomputing each checkpoint segment's stage states and
  * running the reverse stages (hand-written VJPs of cons->prim, WENO5-Z and
- * Rusanov; forward-mode dual numbers for the other functors). */
+ * Rusanov; forward-mode dual numbers for the other functors). Its scratch
+ * (4 + checkpoint_every x stages states and a pinned host copy of one) is
+ * allocated by the first call and kept by the context until mh_destroy. */
 int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, double max_cfl, const double* dts_in,
                        double* dts_out);
 int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_initial);

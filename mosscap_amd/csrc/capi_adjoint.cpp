@@ -24,6 +24,11 @@ StageCoef coef_of(int kind) {
     }
 }
 
+// a device buffer owned elsewhere
+struct DevPtr {
+    double* p = nullptr;
+};
+
 struct DevVec {
     double* p = nullptr;
     hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(double)); }
@@ -111,17 +116,17 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     g.gravity = ctx->gravity;
     g.gx = ctx->grav[0];
     g.gy = ctx->grav[1];
-    // scratch: the x sweep's primitive cotangents and three cotangent states
-    DevVec lamx, acc, cur, nxt;
-    for (DevVec* v : {&lamx, &acc, &cur, &nxt})
-        if (e == hipSuccess) e = v->alloc(nv);
-    // stage states of one segment: (every) steps x ns stages
-    std::vector<DevVec> seg((size_t)every * ns);
-    for (auto& v : seg)
-        if (e == hipSuccess) e = v.alloc(nv);
+    // scratch, allocated by the first reverse sweep and kept by the context:
+    // the x sweep's primitive cotangents, three cotangent states, the stage
+    // states of one segment ((every) steps x ns stages) and pinned host
+    // staging for the cotangents' transfers
+    if (e == hipSuccess) e = ctx->adjoint_scratch(4 + (size_t)every * ns);
     if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_backward: scratch");
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(cur.p, lambda_final, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    DevPtr lamx{ctx->adj_scratch[0]}, acc{ctx->adj_scratch[1]}, cur{ctx->adj_scratch[2]}, nxt{ctx->adj_scratch[3]};
+    std::vector<DevPtr> seg((size_t)every * ns);
+    for (size_t k = 0; k < seg.size(); ++k) seg[k].p = ctx->adj_scratch[4 + k];
+    std::memcpy(ctx->adj_host, lambda_final, nv * sizeof(double));
+    e = hipMemcpyAsync(cur.p, ctx->adj_host, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, cur.p, 0.0, cur.p, ctx->stream);  // clear ghost entries
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&ev0);
@@ -162,8 +167,9 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     }
     if (e == hipSuccess) e = hipEventRecord(ev1, ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(lambda_initial, cur.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        e = hipMemcpyAsync(ctx->adj_host, cur.p, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) std::memcpy(lambda_initial, ctx->adj_host, nv * sizeof(double));
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
     ctx->adj_bwd_ms = e == hipSuccess ? (double)ms : -1.0;

@@ -330,6 +330,29 @@ struct mh_ctx {
     std::vector<double> adj_dts;
     int adj_every = 0;
     double adj_bwd_ms = 0.0;  // device time of the last reverse sweep
+    // reverse-sweep scratch kept between mh_adjoint_backward calls (nvals
+    // doubles each) and pinned host staging for its transfers
+    std::vector<double*> adj_scratch;
+    double* adj_host = nullptr;
+    hipError_t adjoint_scratch(size_t nbuf) {
+        hipError_t e = hipSuccess;
+        if (adj_scratch.size() != nbuf) {
+            free_adjoint_scratch();
+            adj_scratch.assign(nbuf, nullptr);
+            for (size_t k = 0; k < nbuf && e == hipSuccess; ++k)
+                e = hipMalloc(&adj_scratch[k], (size_t)nvals * sizeof(double));
+        }
+        if (e == hipSuccess && !adj_host) e = hipHostMalloc(&adj_host, (size_t)nvals * sizeof(double));
+        if (e != hipSuccess) free_adjoint_scratch();
+        return e;
+    }
+    void free_adjoint_scratch() {
+        for (double* p : adj_scratch)
+            if (p) hipFree(p);
+        adj_scratch.clear();
+        if (adj_host) hipHostFree(adj_host);
+        adj_host = nullptr;
+    }
     void free_adjoint() {
         for (double* p : ckpt)
             if (p) hipFree(p);
@@ -369,6 +392,7 @@ struct mh_ctx {
 
     void free_all() {
         free_adjoint();
+        free_adjoint_scratch();
         for (int k = 0; k < 3; ++k) {
             if (graph[k]) hipGraphExecDestroy(graph[k]);
             graph[k] = nullptr;

@@ -9,8 +9,11 @@
 // a = (d2-d1, d3-d2, d4-d3), b = (3d2-d1, d2+d3, d4-3d3); the common factor
 // 13/12 cancels in tau/(beta+eps) once eps is scaled by 12/13. The ENO
 // candidates, relative to s0 and times 6, are g = (d1-4d2, -2d2-d3, 2d4-5d3)
-// on the left face and h = (4d3-d4, d2+2d3, 5d2-2d1) on the right, so
-// w = s0 + sum(alpha g) / (6 sum(alpha)) with the reference's weights.
+// = (-(b0+d2), -(d2+b1), 2b2+d3) on the left face and h = (4d3-d4, d2+2d3,
+// 5d2-2d1) = (d3-b2, b1+d3, 2b0-d2) on the right. With the reference's
+// weights (3q0, 6q1, q2) on the left and (3q2, 6q1, q0) on the right,
+// w = s0 + sum(alpha g) / (6 sum(alpha)); the 6 is divided into the
+// candidates, so wL = s0 + (q0 g0/2 + q1 g1 + q2 g2/6) / (3q0 + 6q1 + q2).
 //
 // HLLC with (i) c_s = gamma p * rsq(gamma p rho), (ii) the star-state factor
 // sqrt(1 + g (p*/p - 1)) = X rsq(X p) with X = p + g (p* - p), and (iii) only
@@ -35,32 +38,36 @@ __device__ inline void weno5z_f32(const S& s, float& wL, float& wR) {
     const float a0 = d2 - d1, b0 = __builtin_fmaf(3.0f, d2, -d1);
     const float a1 = d3 - d2, b1 = d2 + d3;
     const float a2 = d4 - d3, b2 = __builtin_fmaf(-3.0f, d3, d4);
-    const float be0 = __builtin_fmaf(k * b0, b0, a0 * a0);
-    const float be1 = __builtin_fmaf(k * b1, b1, a1 * a1);
-    const float be2 = __builtin_fmaf(k * b2, b2, a2 * a2);
+    // (12/13) (beta_k + eps): eps, scaled alike, folded into the first product
     constexpr float eps = 1e-15f * (12.0f / 13.0f);
+    const float be0 = __builtin_fmaf(k * b0, b0, __builtin_fmaf(a0, a0, eps));
+    const float be1 = __builtin_fmaf(k * b1, b1, __builtin_fmaf(a1, a1, eps));
+    const float be2 = __builtin_fmaf(k * b2, b2, __builtin_fmaf(a2, a2, eps));
     const float tau = __builtin_fabsf(be0 - be2);
     // tau/(beta+eps) reaches ~1e19 next to a flat stencil; clamped so that
     // its square (and the weighted sums) stay finite in FP32. A weight ratio
     // of 1e30 keeps the ENO choice exact to FP32 precision.
     constexpr float tmax = 1e15f;
-    const float t0 = __builtin_fminf(tau * rcp32(be0 + eps), tmax);
-    const float t1 = __builtin_fminf(tau * rcp32(be1 + eps), tmax);
-    const float t2 = __builtin_fminf(tau * rcp32(be2 + eps), tmax);
+    const float t0 = __builtin_fminf(tau * rcp32(be0), tmax);
+    const float t1 = __builtin_fminf(tau * rcp32(be1), tmax);
+    const float t2 = __builtin_fminf(tau * rcp32(be2), tmax);
     const float q0 = __builtin_fmaf(t0, t0, 1.0f);
     const float q1 = __builtin_fmaf(t1, t1, 1.0f);
     const float q2 = __builtin_fmaf(t2, t2, 1.0f);
-    const float A0 = 3.0f * q0, A1 = 6.0f * q1, A2 = 3.0f * q2;
-    const float g0 = __builtin_fmaf(-4.0f, d2, d1);
-    const float g1 = __builtin_fmaf(-2.0f, d2, -d3);
-    const float g2 = __builtin_fmaf(2.0f, d4, -5.0f * d3);
-    const float h0 = __builtin_fmaf(4.0f, d3, -d4);
-    const float h1 = __builtin_fmaf(2.0f, d3, d2);
-    const float h2 = __builtin_fmaf(5.0f, d2, -2.0f * d1);
-    const float nL = __builtin_fmaf(A0, g0, __builtin_fmaf(A1, g1, q2 * g2));
-    const float nR = __builtin_fmaf(A2, h0, __builtin_fmaf(A1, h1, q0 * h2));
-    wL = __builtin_fmaf(nL, rcp32(6.0f * (A0 + A1 + q2)), s0);
-    wR = __builtin_fmaf(nR, rcp32(6.0f * (A2 + A1 + q0)), s0);
+    // candidates relative to s0 with the weights' common factor 1/6 taken
+    // into them: g0/2, g1, g2/6 (left face) and h0/2, h1, h2/6 (right face)
+    const float g0 = -0.5f * (b0 + d2);
+    const float g1 = -(d2 + b1);
+    const float g2 = __builtin_fmaf(2.0f, b2, d3) * (1.0f / 6.0f);
+    const float h0 = 0.5f * (d3 - b2);
+    const float h1 = b1 + d3;
+    const float h2 = __builtin_fmaf(2.0f, b0, -d2) * (1.0f / 6.0f);
+    const float nL = __builtin_fmaf(q0, g0, __builtin_fmaf(q1, g1, q2 * g2));
+    const float nR = __builtin_fmaf(q2, h0, __builtin_fmaf(q1, h1, q0 * h2));
+    const float sL = __builtin_fmaf(3.0f, q0, __builtin_fmaf(6.0f, q1, q2));
+    const float sR = __builtin_fmaf(3.0f, q2, __builtin_fmaf(6.0f, q1, q0));
+    wL = __builtin_fmaf(nL, rcp32(sL), s0);
+    wR = __builtin_fmaf(nR, rcp32(sR), s0);
 }
 
 // EOS constants of the FP32 functors, rounded once from FP64 on the host so
@@ -135,15 +142,15 @@ __device__ inline void hllc_f32(const Eos32& e, const WL& wL, const WR& wR, FT& 
     const float gpL = gamma * pL, gpR = gamma * pR;
     const float csL = gpL * rsq32(gpL * rL);
     const float csR = gpR * rsq32(gpR * rR);
-    const float EL = energy_f32<NumDim>(e, wL);
-    const float ER = energy_f32<NumDim>(e, wR);
     const float rho_avg = 0.5f * (rL + rR);
     const float cs_avg = 0.5f * (csL + csR);
     const float p_mid = 0.5f * __builtin_fmaf((vL - vR) * rho_avg, cs_avg, pL + pR);
-    const float XL = __builtin_fmaf(gfac, p_mid - pL, pL);
-    const float XR = __builtin_fmaf(gfac, p_mid - pR, pR);
-    const float cLs = (p_mid <= pL) ? 1.0f : XL * rsq32(XL * pL);
-    const float cRs = (p_mid <= pR) ? 1.0f : XR * rsq32(XR * pR);
+    // star-state factor: 1 where p_mid <= p (X <= p, so max(X, p) = p and
+    // p rsq(p p) = 1 to rounding), X rsq(X p) where p_mid > p
+    const float XL = __builtin_fmaxf(__builtin_fmaf(gfac, p_mid - pL, pL), pL);
+    const float XR = __builtin_fmaxf(__builtin_fmaf(gfac, p_mid - pR, pR), pR);
+    const float cLs = XL * rsq32(XL * pL);
+    const float cRs = XR * rsq32(XR * pR);
     const float aL = __builtin_fmaf(-csL, cLs, vL);
     const float aR = __builtin_fmaf(csR, cRs, vR);
     const float bM = (aL < -tiny) ? aL : -tiny;
@@ -155,12 +162,19 @@ __device__ inline void hllc_f32(const Eos32& e, const WL& wL, const WR& wR, FT& 
     const float mR = -(rR * uR);
     const float iD = rcp32(mL + mR);
     const float aM = (tL - tR) * iD;
-    const float cp0 = __builtin_fmaf(mL, tR, mR * tL) * iD;
-    const float cp = (cp0 < 0.0f) ? 0.0f : cp0;
-    // upwind side of the contact
+    const float cp = __builtin_fmaxf(__builtin_fmaf(mL, tR, mR * tL) * iD, 0.0f);
+    // upwind side of the contact: its primitives, then its total energy
     const bool left = aM >= 0.0f;
-    const float rK = left ? rL : rR, pK = left ? pL : pR, vK = left ? vL : vR, EK = left ? EL : ER;
+    const float rK = left ? rL : rR, pK = left ? pL : pR;
     const float bK = left ? bM : bP;
+    float vKa[NumDim];
+#pragma unroll
+    for (int a = 0; a < NumDim; ++a) vKa[a] = left ? wL[1 + a] : wR[1 + a];
+    const float vK = vKa[Axis];
+    float v2 = vKa[0] * vKa[0];
+#pragma unroll
+    for (int a = 1; a < NumDim; ++a) v2 = __builtin_fmaf(vKa[a], vKa[a], v2);
+    const float EK = __builtin_fmaf(pK, e.igm1, 0.5f * rK * v2);
     const float uK = vK - bK;
     const float mfK = rK * uK;
     const float r = rcp32(aM - bK);
@@ -168,10 +182,7 @@ __device__ inline void hllc_f32(const Eos32& e, const WL& wL, const WR& wR, FT& 
     const float sMcp = -bK * r * cp;
     flux[0] = sK * mfK;
 #pragma unroll
-    for (int a = 1; a <= NumDim; ++a) {
-        const float vKa = left ? wL[a] : wR[a];
-        flux[a] = sK * (mfK * vKa);
-    }
+    for (int a = 1; a <= NumDim; ++a) flux[a] = sK * (mfK * vKa[a - 1]);
     flux[IV] = __builtin_fmaf(sK, pK, flux[IV]) + sMcp;
     flux[IE] = __builtin_fmaf(sK, __builtin_fmaf(EK, uK, pK * vK), sMcp * aM);
 }

@@ -130,11 +130,11 @@ struct FormF32c {
     // cons->prim with FP64 pressure recovery. Returns true when a floor fired.
     // The density floor applies before 1/rho is formed, so a state whose
     // density went non-positive still yields finite velocities.
-    template <int ND, int NV>
+    template <int ND, bool Floors = true, int NV>
     __device__ static inline bool to_prim_nd(const Ptrs& p, const float2 (&q)[NV], float (&w)[NV]) {
         double rho_d = df::to_double(q[0]);
         bool hit = false;
-        const bool floors = p.rho_floor > 0.0f;
+        const bool floors = Floors && p.rho_floor > 0.0f;
         if (floors && rho_d < (double)p.rho_floor) {
             rho_d = (double)p.rho_floor;
             hit = true;
@@ -190,7 +190,11 @@ struct FormF32c {
     // increment (dq + S*dt) in FP32.
     __device__ static inline float2 combine(int kind, float2 Q, float2 Qn, float dq_flux, float src) {
 #pragma clang fp contract(off)
-        const float dq = dq_flux + src;
+        return combine_nosrc(kind, Q, Qn, dq_flux + src);
+    }
+    // The same with the increment dq of the flux term alone (no source).
+    __device__ static inline float2 combine_nosrc(int kind, float2 Q, float2 Qn, float dq) {
+#pragma clang fp contract(off)
         const auto around_qn = [&](float a_qs, float b) {
             const float d = (Q.x - Qn.x) + (Q.y - Qn.y);
             return df::add_f(Qn, a_qs * (d + b * dq));
@@ -204,6 +208,8 @@ struct FormF32c {
             default: return around_qn(1.0f / 3.0f, 0.5f);       // 2/3 Q^n + 1/3 Q + 1/6 dq
         }
     }
+    // adds of zero (the flux sum's leading 0, a zero source) are left out
+    static constexpr bool kExactZeros = false;
 
     // Positivity floors of a stage's new state (with the floors on): a
     // density below rho_floor (or NaN) puts the cell at rest at rho_floor and
@@ -213,9 +219,9 @@ struct FormF32c {
     // emptied below zero by its outflow, after which 1/rho of the floored
     // density turns its momentum into an unbounded velocity. Returns true
     // when it changed the cell.
-    template <int ND, int NV>
+    template <int ND, bool Floors = true, int NV>
     __device__ static inline bool floor_state(const Ptrs& p, float2 (&q)[NV]) {
-        if (!(p.rho_floor > 0.0f)) return false;
+        if (!Floors || !(p.rho_floor > 0.0f)) return false;
         const double rho = df::to_double(q[0]);
         const double pf = (double)p.p_floor;
         if (!(rho >= (double)p.rho_floor)) {
@@ -242,11 +248,15 @@ struct FormF32c {
         return false;
     }
 
+    // The floors are on: the stage launchers then pick the instance that
+    // compiles them in (Floors = true); the plain one leaves them out.
+    static inline bool floors_on(const Ptrs& p) { return p.rho_floor > 0.0f; }
+
     // Reconstructed face states get the same floors as the cell states
     // (WENO and PPM may undershoot between positive cell values).
-    template <int ND, int NV>
+    template <int ND, bool Floors = true, int NV>
     __device__ static inline void floor_face(const Ptrs& p, float (&w)[NV]) {
-        if (p.rho_floor > 0.0f) {
+        if (Floors && p.rho_floor > 0.0f) {
             w[0] = w[0] < p.rho_floor ? p.rho_floor : w[0];
             w[1 + ND] = w[1 + ND] < p.p_floor ? p.p_floor : w[1 + ND];
         }

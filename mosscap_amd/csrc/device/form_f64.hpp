@@ -75,17 +75,18 @@ struct FormF64 {
         q[Cons<ND>::Ene] -= square(q[IM] - mom_pre) / q[Cons<ND>::Rho];
     }
 
-    template <int ND, int NV>
+    template <int ND, bool Floors = true, int NV>
     __device__ static inline bool to_prim_nd(const Ptrs& p, const double (&q)[NV], double (&w)[NV]) {
         cons_to_prim<ND>(p.gamma, q, w);
         return false;  // no floors in the drop-in
     }
 
     // no floors in the drop-in (the reference has none)
-    template <int ND, int NV>
+    template <int ND, bool Floors = true, int NV>
     __device__ static inline void floor_face(const Ptrs&, double (&)[NV]) {}
-    template <int ND, int NV>
+    template <int ND, bool Floors = true, int NV>
     __device__ static inline bool floor_state(const Ptrs&, double (&)[NV]) { return false; }
+    static inline bool floors_on(const Ptrs&) { return false; }
 
     // TimeStepping.cpp stage lambdas (RK2 60-67/80-87, SSPRK3 137-144/157-164/
     // 177-184, SSPRK(4,3) 234-301), grouped exactly as written there. Generic
@@ -102,6 +103,14 @@ struct FormF64 {
             default: return T(2.0 / 3.0) * Qn + T(1.0 / 3.0) * Q + T(1.0 / 6.0) * (dq + src);
         }
     }
+
+    // ... without a registered source: S*dt = 0*dt = +0, as the reference adds it
+    template <class T>
+    __device__ static inline T combine_nosrc(int kind, T Q, T Qn, T dq) {
+        return combine(kind, Q, Qn, dq, T(0.0));
+    }
+    // the drop-in keeps every operation of the reference, adds of zeros included
+    static constexpr bool kExactZeros = true;
 
     // Source S*dt of the stage lambdas for a registered gravity term:
     // gravity_kernel (Gravity.cpp:22-38) accumulated into a zeroed S from the

@@ -4,14 +4,15 @@
 // rows. Lane t holds column c = i0 - 1 + t (the 62 outputs are lanes 1..62,
 // lanes 0 and 63 supply the outer reconstructions and faces). Per output row j:
 //
-//   1. load row j+1+R of Q^(s) (coalesced, x fastest), cons->prim, write the
-//      primitive row into a 4-row LDS ring (x stencils) and append the lane's
-//      own column value to a (2R+1)-row register window (y stencil);
+//   1. load row j+1+R of Q^(s) (coalesced, x fastest), cons->prim, and append
+//      the lane's own column value to a (2R+2)-row register window (the y
+//      stencil of cell j+1 and row j);
 //   2. y sweep: reconstruct cell j+1 from the register window, Riemann flux on
 //      face j+1 from (RR of cell j, RL of cell j+1), dy = Gy_j - Gy_{j+1};
-//   3. x sweep: reconstruct row j from the LDS ring, take RR of the left
-//      neighbour lane, Riemann flux on the face, dx = Fx_c - Fx_{c+1} (flux of
-//      the right neighbour lane);
+//   3. x sweep: row j's primitives (and the halo columns) go to a one-row LDS
+//      exchange; reconstruct row j from it, take RR of the left neighbour
+//      lane, Riemann flux on the face, dx = Fx_c - Fx_{c+1} (flux of the
+//      right neighbour lane);
 //   4. combine in registers: dq = (0 + dx) + dy, dq *= dt/dx, stage update with
 //      Q^(s) and Q^n, store Q^(s+1); optionally fold |v|+c_s into a running
 //      max for the next CFL time step.
@@ -65,21 +66,26 @@ struct StageGrid {
 // interior z-plane per (strip, span, plane) workgroup; the z flux difference
 // of each cell comes from the z sweep (p.dz3) and is added after the y one,
 // as integrate_flux adds Fz after Fy (TimeStepping.cpp:22-30).
-// Extras = false compiles out the gravity source and the tracers' density-flux
-// stores (their addresses and operands cost registers the plain hydro stage
-// needs); the launcher picks the instance from the pointers it is given.
+// Extras = false compiles out the gravity source, the tracers' density-flux
+// stores and the positivity floors (their addresses, operands and branches
+// cost registers the plain hydro stage needs); the launcher picks the
+// instance from the pointers and options it is given.
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int ND = 2, bool Extras = true>
 __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Ptrs& p) {
     using T = typename Form::real;
     using Raw = typename Form::raw;
     constexpr int NV = 2 + ND;  // rho, m_x, m_y[, m_z], E
     constexpr int ORD = stencil_order(R);
-    constexpr int WIN = 2 * ORD + 1;
+    // register window of the lane's own column: rows j+1-ORD .. j+1+ORD (the
+    // y stencil of cell j+1), which holds row j (whose x sweep runs this
+    // iteration) except for FOG, whose window reaches one row further back
+    constexpr int WIN = 2 * ORD + 1 + (ORD == 0 ? 1 : 0);
+    constexpr int WY = WIN - 1 - ORD;  // window index of cell j+1
+    constexpr int WX = WY - 1;         // ... and of row j
     constexpr int ROW = kWave + 2 * ORD;
-    constexpr int RING = 4;  // rows j .. j+1+ORD are live
-    static_assert(ORD + 2 <= RING, "ring too shallow");
 
-    __shared__ T ring[RING][NV][ROW];
+    // row j's primitives with the halo columns, exchanged for the x stencils
+    __shared__ T xrow[NV][ROW];
 
     const int lane = threadIdx.x;
     const uint32_t bb = xcd_remap(blockIdx.x, gridDim.x);
@@ -101,11 +107,13 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const int plane = g.xc * g.yc * (ND > 2 ? g.zc : 1);
     const int pk = ND > 2 ? (g.ng + (int)(bb / per_plane)) * g.xc * g.yc : 0;
 
-    // Halo columns: ring entries [0, ORD) (columns left of the strip) and
-    // [64+ORD, 64+2*ORD) (right of it). They are converted to primitives in
-    // batches of HB rows, one lane per (row, column) pair, into `hbuf`, and
-    // lanes [0, 2*ORD) copy a committed row's entries into the ring: one
-    // cons->prim pass per HB rows instead of a divergent pass per row.
+    // Halo columns: exchange-row entries [0, ORD) (columns left of the strip)
+    // and [64+ORD, 64+2*ORD) (right of it). They are converted to primitives
+    // in batches of HB rows, one lane per (row, column) pair, into `hbuf`
+    // (two batches deep: a batch is converted when its first row is
+    // committed, up to ORD+1 rows before that row's x sweep), and lanes
+    // [0, 2*ORD) copy a row's entries into the exchange row: one cons->prim
+    // pass per HB rows instead of a divergent pass per row.
     constexpr int NH = 2 * ORD;
     // 4 rows: the batch reads at most 3 rows ahead of the neighbouring strips'
     // waves, whose interior these columns are, so the lines are still in L2
@@ -114,7 +122,11 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     constexpr int HB = NH > 0 ? 4 : 1;
     static_assert(HB * NH <= kWave, "one lane per (row, column) of a batch");
     static_assert((HB & (HB - 1)) == 0, "batch rows must be a power of two");
-    __shared__ T hbuf[HB][NV][NH > 0 ? NH : 1];
+    constexpr int HSLOTS = 2 * HB;
+    // the batch holding row j + HSLOTS is converted at iteration j + HSLOTS -
+    // (HB-1) - 1 - ORD or later, not before row j's x sweep (iteration j)
+    static_assert(ORD <= HSLOTS - HB, "halo batches overwritten before their rows are swept");
+    __shared__ T hbuf[HSLOTS][NV][NH > 0 ? NH : 1];
     const bool halo_lane = lane < NH;
     const int h_entry = lane < ORD ? lane : kWave + lane;
     const int r_first = j0 - 1 - ORD;  // first and last rows the march commits
@@ -124,6 +136,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     // for the wave-uniform row/plane/variable part, fixed per-lane vector
     // offsets (the launcher keeps every byte offset below 2^31).
     constexpr bool kBuf = !Extras;
+    // positivity floors (the compensated form's option) live in the other
+    // instance too; the launchers pick it when they are on
+    constexpr bool kFloors = Extras;
     const auto bqs = Form::buf(p.qs);
     const auto bqn = Form::buf(p.qn);
     const auto bout = Form::buf(p.out);
@@ -137,7 +152,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     const T dtdx = dt / (T)p.dx;
     const T src = T(0) * dt;
 
-    T win[WIN][NV];
+    T win[WIN][NV] = {};
     const bool out_lane = lane >= 1 && lane <= kStripOut && c < g.ng + g.nx;
 
     // Plain instance: the stage-combine operands of the next row (Q^(s) and
@@ -164,13 +179,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     uint32_t in_floor_bits = 0;
 
     auto commit_row = [&](int r, const Raw (&q)[NV]) {
-        const int slot = r & (RING - 1);
-        // halo entries of row r: read from the batch first, so that the LDS
-        // latency hides behind this row's cons->prim
-        T hv[NV];
         if constexpr (NH > 0) {
-            const int hr = (r - r_first) & (HB - 1);
-            if (hr == 0) {  // wave-uniform: convert the halo of rows r .. r+HB-1
+            if (((r - r_first) & (HB - 1)) == 0) {  // wave-uniform: convert the halo of rows r .. r+HB-1
                 wave_lds_sync();  // the previous batch's reads are done
                 const int hb_idx = lane % NH;  // this lane's halo column
                 const int hb_row = lane / NH;  // and row offset in the batch
@@ -187,28 +197,16 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
                     for (int v = 0; v < NV; ++v) qh[v] = load_qs(rs * xc + hb_col, v * plane + pk);
                     T wh[NV];
-                    Form::template to_prim_nd<ND>(p, qh, wh);
+                    Form::template to_prim_nd<ND, kFloors>(p, qh, wh);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) hbuf[hb_row][v][hb_idx] = wh[v];
+                    for (int v = 0; v < NV; ++v) hbuf[(rr - r_first) & (HSLOTS - 1)][v][hb_idx] = wh[v];
                 }
                 wave_lds_sync();
             }
-            if (halo_lane) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) hv[v] = hbuf[hr][v][lane];
-            }
         }
         T w[NV];
-        const bool hit = Form::template to_prim_nd<ND>(p, q, w);
+        const bool hit = Form::template to_prim_nd<ND, kFloors>(p, q, w);
         in_floor_bits = (in_floor_bits << 1) | (hit ? 1u : 0u);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) ring[slot][v][lane + ORD] = w[v];
-        if constexpr (NH > 0) {
-            if (halo_lane) {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) ring[slot][v][h_entry] = hv[v];
-            }
-        }
 #pragma unroll
         for (int k = 0; k + 1 < WIN; ++k)
 #pragma unroll
@@ -224,11 +222,11 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     auto recon_y = [&](T (&wl)[NV], T (&wr)[NV]) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            auto s = [&](int o) -> T { return win[ORD + o][v]; };
+            auto s = [&](int o) -> T { return win[WY + o][v]; };
             reconstruct_stencil<R, SL, T>(s, wl[v], wr[v]);
         }
-        Form::template floor_face<ND>(p, wl);
-        Form::template floor_face<ND>(p, wr);
+        Form::template floor_face<ND, kFloors>(p, wl);
+        Form::template floor_face<ND, kFloors>(p, wr);
     };
 
     // ---- prologue: rows j0-1-ORD .. j0+ORD; y faces up to j0 ------------------
@@ -277,7 +275,6 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             }
         };
         if constexpr (Form::kEarlyOperands && !kDmaOps) load_operands();
-        wave_lds_sync();  // earlier ring reads complete before the slot is reused
         commit_row(j + 1 + ORD, qa);
         if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
 
@@ -293,17 +290,28 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             yRp[v] = yr[v];
         }
 
-        wave_lds_sync();  // row j of the ring visible to every lane
-        // x sweep on row j
-        const int slot = j & (RING - 1);
+        // x sweep on row j: its primitives (window row WX) and halo entries
+        // (their batch was converted ORD+1 or more rows ago) go to the
+        // exchange row, which the previous row's x sweep has read (one
+        // wavefront: its LDS operations complete in program order)
+        wave_lds_sync();
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xrow[v][lane + ORD] = win[WX][v];
+        if constexpr (NH > 0) {
+            if (halo_lane) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xrow[v][h_entry] = hbuf[(j - r_first) & (HSLOTS - 1)][v][lane];
+            }
+        }
+        wave_lds_sync();  // row j visible to every lane
         T xl[NV], xr[NV], xrm[NV], Fx[NV], Fxp[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            auto s = [&](int o) -> T { return ring[slot][v][lane + ORD + o]; };
+            auto s = [&](int o) -> T { return xrow[v][lane + ORD + o]; };
             reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
         }
-        Form::template floor_face<ND>(p, xl);
-        Form::template floor_face<ND>(p, xr);
+        Form::template floor_face<ND, kFloors>(p, xl);
+        Form::template floor_face<ND, kFloors>(p, xr);
 #pragma unroll
         for (int v = 0; v < NV; ++v) xrm[v] = lane_from_left(xr[v]);
         riemann_flux<RS, 0, ND>(eos, xrm, xl, Fx);
@@ -332,15 +340,17 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             }
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                T q_update = T(0);
-                q_update += Fx[v] - Fxp[v];
+                // integrate_flux's 0 + dx + dy (+ dz) (the compensated form
+                // drops the leading 0, which only turns -0 into +0)
+                T q_update = Form::kExactZeros ? T(0) + (Fx[v] - Fxp[v]) : Fx[v] - Fxp[v];
                 q_update += dy[v];
                 if constexpr (ND > 2) q_update += p.dz3[v * plane + pk + j * xc + c];
                 q_update *= dtdx;
-                qo[v] = Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v]);
+                qo[v] = (Extras && p.gravity) ? Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v])
+                                              : Form::combine_nosrc(g.kind, qs_c[v], qn_c[v], q_update);
             }
             // each interior cell is combined by exactly one (strip, span) owner
-            const bool st_hit = Form::template floor_state<ND>(p, qo);
+            const bool st_hit = Form::template floor_state<ND, kFloors>(p, qo);
             floor_hits += (st_hit || ((in_floor_bits >> (ORD + 1)) & 1u)) ? 1u : 0u;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -440,10 +450,10 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            T q_update = T(0);
-            q_update += Fx[v] - Fxp[v];
+            T q_update = Form::kExactZeros ? T(0) + (Fx[v] - Fxp[v]) : Fx[v] - Fxp[v];
             q_update *= dtdx;
-            qo[v] = Form::combine(kind, q[v], qn[v], q_update, srcv[v]);
+            qo[v] = p.gravity ? Form::combine(kind, q[v], qn[v], q_update, srcv[v])
+                              : Form::combine_nosrc(kind, q[v], qn[v], q_update);
         }
         const bool st_hit = Form::template floor_state<1>(p, qo);
         floor_hits += (in_hit || st_hit) ? 1u : 0u;

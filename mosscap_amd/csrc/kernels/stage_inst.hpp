@@ -99,11 +99,12 @@ hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typen
         g.yc != g.ny + 2 * g.ng || g.span_h < 1 || g.strips != (g.nx + kStripOut - 1) / kStripOut) {
         return hipErrorInvalidValue;
     }
-    // the plain instance (buffer-resource addressing, no source or tracer
-    // stores) unless a gravity source or tracer flux stores are asked for, or
-    // the state's byte offsets do not fit the buffer offsets' 31 bits
+    // the plain instance (buffer-resource addressing, no source, tracer
+    // stores or floors) unless a gravity source, tracer flux stores or
+    // positivity floors are asked for, or the state's byte offsets do not fit
+    // the buffer offsets' 31 bits
     const bool big = (int64_t)4 * g.xc * g.yc * Form::kElem >= ((int64_t)1 << 31);
-    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr || big;
+    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr || Form::floors_on(p) || big;
     auto pick = [&]<Reconstruction R, SlopeLimiter SL>() {
         return extras ? SolverPick<Form, R, SL>::template two<true>(s.solver)
                       : SolverPick<Form, R, SL>::template two<false>(s.solver);
@@ -156,10 +157,10 @@ hipError_t launch_stage3d_impl(const Schemes& s, const StageGrid& g, const typen
             default: return zsweep_kernel<Form, R, SL, RiemannSolver::Hllc>;
         }
     };
-    // plain instance (buffer addressing, no source) unless gravity is on or a
-    // byte offset could pass 2^31
+    // plain instance (buffer addressing, no source or floors) unless gravity
+    // or the floors are on or a byte offset could pass 2^31
     const bool big = (int64_t)5 * g.xc * g.yc * g.zc * Form::kElem >= ((int64_t)1 << 31);
-    const bool extras = p.gravity != 0 || big;
+    const bool extras = p.gravity != 0 || Form::floors_on(p) || big;
     auto mpick = [&]<Reconstruction R, SlopeLimiter SL>() -> Stage2dFn<Form> {
         switch (s.solver) {
             case RiemannSolver::Rusanov:
