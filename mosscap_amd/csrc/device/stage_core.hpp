@@ -219,6 +219,18 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = load_qs(c_ld, v * plane + base);
     };
+    // Plain instance: the next stencil row of Q^(s) goes straight into LDS
+    // too (issued at the top of a row, complete by its combine, which waits
+    // for every vector-memory load), so no registers hold it across a row.
+    __shared__ uint32_t lead[kDmaOps ? NV : 1][2][kWave];  // [var][word][lane]
+    auto issue_lead = [&](int r) {
+        if constexpr (kDmaOps) {
+            const int base = pk + src_row(r) * xc;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                Form::dma(bqs, &lead[v][0][0], &lead[v][1][0], (uint32_t)c_ld * ES, (uint32_t)(v * plane + base));
+        }
+    };
     auto recon_y = [&](T (&wl)[NV], T (&wr)[NV]) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -249,8 +261,15 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         if (Extras && ND == 2 && p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
 #pragma unroll
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
-        if (j0 < j1) fetch_row(j0 + 1 + ORD, qa);
-        if (j0 < j1) issue_operands(j0);
+        if constexpr (kDmaOps) {
+            if (j0 < j1) {
+                issue_lead(j0 + 1 + ORD);
+                issue_operands(j0);
+                wait_vm_loads();  // the first row reads both without a combine's wait before it
+            }
+        } else {
+            if (j0 < j1) fetch_row(j0 + 1 + ORD, qa);
+        }
     }
 
     double amax = 0.0;
@@ -275,8 +294,21 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             }
         };
         if constexpr (Form::kEarlyOperands && !kDmaOps) load_operands();
-        commit_row(j + 1 + ORD, qa);
-        if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
+        if constexpr (kDmaOps) {
+            wave_lds_sync();
+            Raw ql[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ql[v] = Form::from_words(lead[v][0][lane], lead[v][1][lane]);
+            commit_row(j + 1 + ORD, ql);
+            if (j + 1 < j1) {
+                wait_lds_ops();  // the lead words are read before the next row's overwrite them
+                wave_lds_sync();
+                issue_lead(j + 2 + ORD);
+            }
+        } else {
+            commit_row(j + 1 + ORD, qa);
+            if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
+        }
 
         // y sweep: cell j+1, face j+1
         T yl[NV], yr[NV], Gy[NV], dy[NV];
