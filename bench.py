@@ -208,6 +208,44 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_bytes
     return 0
 
 
+def trace_summary(trace_csv):
+    """Per-kernel launch statistics from a rocprofv3 kernel trace (CSV), grouped by kernel and grid
+    size (launch shapes, e.g. the autotuner's trial spans, are kept apart), and the gaps between
+    consecutive launches inside graph-replayed runs (launches less than 50 us apart)."""
+    import csv
+    from collections import defaultdict
+    rows = list(csv.DictReader(open(trace_csv)))
+    if not rows:
+        print("empty trace")
+        return 1
+    grid_key = next((k for k in ("Grid_Size_X", "Grid_Size", "grid_size_x") if k in rows[0]), None)
+    recs = []
+    for r in rows:
+        t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        recs.append((t0, t1, r["Kernel_Name"], r.get(grid_key, "?") if grid_key else "?"))
+    recs.sort()
+    by = defaultdict(list)
+    for t0, t1, name, grid in recs:
+        by[(name, grid)].append((t1 - t0) / 1e3)
+    print("| kernel | grid | launches | avg (us) | min (us) | max (us) |")
+    print("|---|---|---|---|---|---|")
+    for (name, grid), d in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        short = name.replace("mh::", "").replace("(mh::Reconstruction)", "R").replace("(mh::SlopeLimiter)", "SL")
+        short = short.replace("(mh::RiemannSolver)", "RS").split("(mh::StageGrid")[0][:90]
+        print(f"| `{short}` | {grid} | {len(d)} | {sum(d) / len(d):.1f} | {min(d):.1f} | {max(d):.1f} |")
+    gaps = defaultdict(list)
+    for (a0, a1, an, _), (b0, b1, bn, _) in zip(recs, recs[1:]):
+        g = (b0 - a1) / 1e3
+        if 0 <= g < 50:
+            form = "FormF64" if "FormF64" in an else "FormF32c" if "FormF32c" in an else None
+            if form:
+                gaps[form].append(g)
+    for form, g in gaps.items():
+        print(f"{form}: {len(g)} inter-launch gaps under 50 us, mean {sum(g) / len(g):.2f} us, "
+              f"max {max(g):.2f} us")
+    return 0
+
+
 def measured_traffic(kernel_prefix):
     """Average HBM bytes per launch of the matching kernel from the committed PMC summary."""
     try:
@@ -357,6 +395,8 @@ def main(argv=None):
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="summarise rocprofv3 --pmc CSVs into the traffic JSON (CPU only) and exit")
     ap.add_argument("--copy-bytes", type=int, default=512 << 20)
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="summarise a rocprofv3 --kernel-trace CSV (CPU only) and exit")
     ap.add_argument("--workload", choices=["kh", "batch", "sedov", "blast3d"], default="kh",
                     help="kh: the headline line; batch / sedov: the other workloads; blast3d: the 3-D path")
     ap.add_argument("--n3", type=int, default=256, help="cells per side of the 3-D blast")
@@ -372,6 +412,8 @@ def main(argv=None):
         return bench_sedov(a)
     if a.pmc_summary:
         return pmc_summary(*a.pmc_summary, a.copy_bytes, TRAFFIC_JSON)
+    if a.trace_summary:
+        return trace_summary(a.trace_summary)
     if a.copy_probe_mb:
         return copy_probe(a.copy_probe_mb)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

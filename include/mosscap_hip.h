@@ -115,10 +115,10 @@ int mh_compute_dt(mh_ctx* ctx, double max_cfl, double time, double max_time, dou
                   double delta, double* dt_out);
 
 /* sim.time_step(sim, dt), TimeStepping.cpp:39-308: every stage, ghost fill
- * after each, ending with ghosts filled. In the compensated form, on a 2-D or
- * 3-D grid whose faces are all periodic or zerograd (no user_bc hook, no
- * tracers, no tabulated EOS), the stages read ghost cells from the cells they
- * copy instead, and one fill at the end of the step leaves the same values. */
+ * after each, ending with ghosts filled. On a 2-D or 3-D grid whose faces are
+ * all periodic or zerograd (no user_bc hook, no tracers, no tabulated EOS),
+ * the stages read ghost cells from the cells they copy instead, and one fill
+ * at the end of the step leaves the same values (the same bits in FP64). */
 int mh_time_step(mh_ctx* ctx, double dt);
 
 /* One stage of the scheme (stage 0 snapshots Q^n; later stages use it);

@@ -83,15 +83,15 @@ struct mh_batch {
             }
     }
 
-    // Compensated form with periodic or zerograd faces: the persistent stage
-    // reads ghosts from their source cells, so the stages of a step fill no
-    // ghosts and mh_batch_run fills the final state's once (the ghosts would
-    // add nothing to the per-problem CFL max: they copy interior values).
+    // Periodic or zerograd faces: the persistent stage reads ghosts from
+    // their source cells, so the stages of a step fill no ghosts and
+    // mh_batch_run fills the final state's once (the ghosts would add nothing
+    // to the per-problem CFL max: they copy interior values).
     bool virtual_ghosts() const {
         const auto copy_bc = [](int32_t x) {
             return x == (int32_t)BoundaryType::Periodic || x == (int32_t)BoundaryType::ZeroGrad;
         };
-        return fp32c() && copy_bc(bdry.xs) && copy_bc(bdry.xe);
+        return copy_bc(bdry.xs) && copy_bc(bdry.xe);
     }
 
     // one stage for every problem: persistent (mode 0) or one launch per problem (mode 1)
@@ -112,7 +112,8 @@ struct mh_batch {
             } else {
                 FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), d_dt, 0.0, gamma, d_amax, nullptr};
                 e = launch_batch_stage_f64(sch, g, grid, p, b, stream);
-                if (e == hipSuccess) e = launch_batch_ghost_f64(ghost_grid(reduce), nprob, p64(in), p64(out), b, stream);
+                if (e == hipSuccess && !g.virt)
+                    e = launch_batch_ghost_f64(ghost_grid(reduce), nprob, p64(in), p64(out), b, stream);
             }
             return e;
         }
@@ -358,7 +359,10 @@ int mh_batch_run(mh_batch* b, int64_t nsteps, double max_cfl, double max_time, i
         b->steps += 1;
     }
     if (e == hipSuccess && mode == 0 && nsteps > 0 && b->virtual_ghosts())  // the final state's ghosts
-        e = launch_batch_ghost_f32c(b->ghost_grid(0), b->nprob, b->p32(b->cur), b->p32(b->cur), arr, b->stream);
+        e = b->fp32c() ? launch_batch_ghost_f32c(b->ghost_grid(0), b->nprob, b->p32(b->cur), b->p32(b->cur), arr,
+                                                 b->stream)
+                       : launch_batch_ghost_f64(b->ghost_grid(0), b->nprob, b->p64(b->cur), b->p64(b->cur), arr,
+                                                b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     return e == hipSuccess ? MH_OK : b->hip_fail(e, "mh_batch_run");
 }

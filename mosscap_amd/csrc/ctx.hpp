@@ -245,21 +245,22 @@ struct mh_ctx {
         return ghost(src, out, reduce);
     }
 
-    // Compensated form on 2-D and 3-D grids whose faces are all periodic or
-    // zerograd, with no user_bc hook, no tracer fields and no tabulated EOS
-    // (its update runs over the ghosts of each stage's input): every ghost is
-    // a copy of an interior cell, so the stage kernels read ghost rows,
-    // columns and planes from their sources (StageGrid::virt) and a step
-    // needs one ghost fill, at its end, instead of one per stage. Leaving the
+    // 2-D and 3-D grids whose faces are all periodic or zerograd, with no
+    // user_bc hook, no tracer fields and no tabulated EOS (its update runs
+    // over the ghosts of each stage's input): every ghost is a copy of an
+    // interior cell, so the stage kernels read ghost rows, columns and planes
+    // from their sources (StageGrid::virt) and a step needs one ghost fill,
+    // at its end, instead of one per stage (each a graph node and a kernel
+    // boundary). A stage reads the values the reference's fill would have
+    // written (Boundaries.hpp), so the results are the same bits; leaving the
     // ghosts out of the CFL max changes nothing (they copy interior values).
-    // The FP64 drop-in keeps the
-    // reference's fill after every stage (Hydro.cpp, TimeStepping.cpp), so its
-    // stage sequence, and the adjoint that recomputes it, stay the reference's.
+    // mh_stage still fills after every stage, and the adjoint's
+    // recomputation fills as the reference does.
     bool virtual_ghosts() const {
         const auto copy_bc = [](int32_t b) {
             return b == (int32_t)BoundaryType::Periodic || b == (int32_t)BoundaryType::ZeroGrad;
         };
-        return fp32c() && nd >= 2 && !user_fn && ntr == 0 && !tab_on && copy_bc(bdry.xs) && copy_bc(bdry.xe) &&
+        return nd >= 2 && !user_fn && ntr == 0 && !tab_on && copy_bc(bdry.xs) && copy_bc(bdry.xe) &&
                copy_bc(bdry.ys) && copy_bc(bdry.ye) && (nd < 3 || (copy_bc(bdry.zs) && copy_bc(bdry.ze)));
     }
     void set_virt(StageGrid& g) const {

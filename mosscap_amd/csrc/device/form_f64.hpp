@@ -37,8 +37,9 @@ struct FormF64 {
     struct Buf {
         __amdgpu_buffer_rsrc_t p;
     };
-    // the drop-in fills ghosts after every stage, as the reference does
-    static constexpr bool kVirtualGhosts = false;
+    // grids with copy-type faces read ghost rows and columns from their
+    // sources instead of filling them after every stage (StageGrid::virt)
+    static constexpr bool kVirtualGhosts = true;
     // the instance without buffer resources loads Q^(s) and Q^n for the
     // combine just before it (fewer registers live across the sweeps)
     static constexpr bool kEarlyOperands = false;

@@ -138,23 +138,25 @@ def test_fp32_functors_close_to_fp64(orc):
 NSTAGES = {"rk2": 2, "ssprk3": 3, "ssprk4": 4}
 
 
-@pytest.mark.parametrize("bcx,bcy,recon,stepper", [
-    ("periodic", "periodic", "weno5z", "ssprk3"),
-    ("zerograd", "zerograd", "weno5z", "ssprk3"),
-    ("periodic", "zerograd", "weno5z", "ssprk3"),
-    ("zerograd", "periodic", "weno5z", "ssprk3"),
-    ("periodic", "zerograd", "weno5z", "rk2"),
-    ("zerograd", "periodic", "weno5z", "ssprk4"),
-    ("periodic", "zerograd", "muscl", "ssprk4"),
-    ("zerograd", "periodic", "ppm", "rk2"),
+@pytest.mark.parametrize("bcx,bcy,recon,stepper,form", [
+    ("periodic", "periodic", "weno5z", "ssprk3", M.FP32C),
+    ("zerograd", "zerograd", "weno5z", "ssprk3", M.FP32C),
+    ("periodic", "zerograd", "weno5z", "ssprk3", M.FP32C),
+    ("zerograd", "periodic", "weno5z", "ssprk3", M.FP32C),
+    ("periodic", "zerograd", "weno5z", "rk2", M.FP32C),
+    ("zerograd", "periodic", "weno5z", "ssprk4", M.FP32C),
+    ("periodic", "zerograd", "muscl", "ssprk4", M.FP32C),
+    ("zerograd", "periodic", "ppm", "rk2", M.FP32C),
+    ("periodic", "zerograd", "weno5z", "ssprk3", M.FP64),
+    ("zerograd", "periodic", "ppm", "ssprk4", M.FP64),
 ])
-def test_ghosts_read_from_sources_equal_filled_ghosts(bcx, bcy, recon, stepper):
-    """On grids whose faces are all periodic or zerograd the compensated form reads ghost rows and
-    columns from their source cells and fills ghosts once per step. A user_bc hook that does
-    nothing switches that off (every stage fills its ghosts); both runs, the graph-replayed loop,
-    the loop without graphs, mh_time_step and mh_stage must give the same bits. Mixed faces (x
-    periodic, y zerograd and the transpose) catch a swapped axis; RK2 and SSPRK(4,3) rotate the
-    buffers differently from SSPRK3."""
+def test_ghosts_read_from_sources_equal_filled_ghosts(bcx, bcy, recon, stepper, form):
+    """On grids whose faces are all periodic or zerograd the stages read ghost rows and columns
+    from their source cells and fill ghosts once per step. A user_bc hook that does nothing
+    switches that off (every stage fills its ghosts); both runs, the graph-replayed loop, the loop
+    without graphs, mh_time_step and mh_stage must give the same bits. Mixed faces (x periodic,
+    y zerograd and the transpose) catch a swapped axis; RK2 and SSPRK(4,3) rotate the buffers
+    differently from SSPRK3."""
     cfg = problems.kh_config(128)
     cfg.bc = (bcx, bcx, bcy, bcy)
     cfg.reconstruction, cfg.time_stepper = recon, stepper
@@ -162,7 +164,7 @@ def test_ghosts_read_from_sources_equal_filled_ghosts(bcx, bcy, recon, stepper):
     q0 = problems.kh_band(cfg)
     out = {}
     for mode in ("graph", "loop", "hook", "steps", "stages"):
-        with HydroContext(cfg, precision_form=M.FP32C) as ctx:
+        with HydroContext(cfg, precision_form=form) as ctx:
             if mode == "hook":
                 ctx.set_user_bc(lambda view: None)
             ctx.upload(q0)
