@@ -64,7 +64,10 @@ typedef struct {
  * options select FP64 and the tuned defaults. */
 typedef struct {
     int32_t precision_form; /* MH_FORM_* */
-    int32_t span_rows;      /* rows marched per wavefront in 2-D; 0 = tuned default */
+    int32_t span_rows;      /* rows marched per wavefront in 2-D (the rows are split into
+                             * ceil(ny / span_rows) spans whose heights differ by at most
+                             * one); <= 0: the default layout, the most spans whose
+                             * wavefronts fit the GPU in one round at 4 per SIMD */
     float rho_floor;        /* FP32C positivity floors on the reconstruction input (cells and
                              * faces) and on each stage's new state (a cell below rho_floor is
                              * put at rest at (rho_floor, p_floor); one below p_floor gets its
@@ -157,9 +160,11 @@ int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, doub
 
 /* Autotuner (no counterpart in the reference): times `steps` device steps of
  * the current state for each candidate span height (rows marched per
- * wavefront in 2-D; 0-terminated list, NULL = built-in candidates) with HIP
- * events, keeps the fastest for this context and returns it in *best_span
- * (and the per-candidate milliseconds per step in ms_out when non-NULL). The
+ * wavefront in 2-D, as mh_options.span_rows; -1 the default layout;
+ * 0-terminated list, NULL = built-in candidates, the default layout
+ * included) with HIP events, keeps the fastest for this context and returns
+ * it in *best_span (and the per-candidate milliseconds per step in ms_out
+ * when non-NULL). The
  * state, the tabulated EOS's fields and the floor-hit count are restored
  * afterwards; the record of the last stage's input is cleared (the trials
  * overwrote that buffer), so mh_eos_fields(which = 1) then reads the current

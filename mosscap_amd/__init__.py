@@ -390,8 +390,9 @@ class HydroContext:
         return t.value, p.value, tm
 
     def autotune(self, steps: int = 3, candidates=None):
-        """Pick the span height (rows per wavefront) by timing; returns (best, {span: ms/step})."""
-        cand = np.array(list(candidates or [4, 8, 12, 16, 18, 20, 24, 28, 32, 48, 64]) + [0], dtype=np.int32)
+        """Pick the span height (rows per wavefront; -1 the default layout) by timing; returns
+        (best, {span: ms/step})."""
+        cand = np.array(list(candidates or [4, 8, 12, 16, 18, 20, 24, 28, 32, 48, 64, -1]) + [0], dtype=np.int32)
         ms = np.zeros(len(cand))
         best = C.c_int32()
         self._c(lib().mh_autotune(self._h, steps, _ptr(cand), C.byref(best), _ptr(ms)))

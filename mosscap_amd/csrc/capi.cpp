@@ -105,7 +105,7 @@ int mh_create(const mh_grid_size* sz, double dx, const mh_grid_loc* loc, const m
     c->ncells = ncells;
     c->nvals = ncells * nvar;
     c->strips = (nx + kStripOut - 1) / kStripOut;
-    c->span_h = c->opt.span_rows > 0 ? c->opt.span_rows : default_span_rows(c->strips, ny, nz);
+    c->nspans = march_spans(c->opt.span_rows, c->strips, ny, nz);
 
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -533,7 +533,8 @@ int mh_run_timed(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, doub
 int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* best_span, double* ms_out) {
     if (!ctx || !best_span || steps < 1) return MH_ERR_INVALID;
     if (ctx->next_stage != 0) return ctx->fail(MH_ERR_INVALID, "a step is in progress (mh_stage)");
-    static const int32_t kDefault[] = {4, 8, 12, 16, 18, 20, 24, 28, 32, 48, 64, 0};
+    // rows per span; -1: the default layout (march_spans with rows <= 0)
+    static const int32_t kDefault[] = {4, 8, 12, 16, 18, 20, 24, 28, 32, 48, 64, -1, 0};
     const int32_t* cand = candidates ? candidates : kDefault;
     hipError_t e = hipSetDevice(ctx->device);
     // keep the state in a scratch copy and restore it after each trial
@@ -573,17 +574,18 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
     if (e == hipSuccess)
         e = hipMemcpyAsync(&keep_hits, ctx->floor_hits, sizeof(keep_hits), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    const int old_span = ctx->span_h;
+    const int old_spans = ctx->nspans;
     const int64_t old_steps = ctx->steps;
     double best = 1e300;
-    int best_h = old_span;
+    int best_h = 0, best_spans = old_spans;
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
     const int ny = ctx->nd > 1 ? ctx->sz.yc - 2 * ctx->sz.ng : 1;
-    for (int k = 0; cand[k] > 0 && e == hipSuccess; ++k) {
+    const int nz = ctx->nd > 2 ? ctx->sz.zc - 2 * ctx->sz.ng : 1;
+    for (int k = 0; cand[k] != 0 && e == hipSuccess; ++k) {
         if (cand[k] > ny && k > 0) continue;
-        ctx->span_h = cand[k];
+        ctx->nspans = march_spans(cand[k], ctx->strips, ny, nz);
         for (int g = 0; g < 3; ++g) {  // graphs bake the launch shape in
             if (ctx->graph[g]) hipGraphExecDestroy(ctx->graph[g]);
             ctx->graph[g] = nullptr;
@@ -605,6 +607,7 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
         if (per < best) {
             best = per;
             best_h = cand[k];
+            best_spans = ctx->nspans;
         }
         ctx->cur = start;
         if (e == hipSuccess) e = stash(true);
@@ -635,10 +638,10 @@ int mh_autotune(mh_ctx* ctx, int32_t steps, const int32_t* candidates, int32_t* 
     ctx->cur = start;
     ctx->amax_valid = false;  // the slots hold a trial state's maximum
     if (e != hipSuccess) {
-        ctx->span_h = old_span;
+        ctx->nspans = old_spans;
         return ctx->hip_fail(e, "mh_autotune");
     }
-    ctx->span_h = best_h;
+    ctx->nspans = best_spans;
     *best_span = best_h;
     return MH_OK;
 }

@@ -131,7 +131,7 @@ int mh_step_noalloc(const mh_grid_size* sz, double dx, const mh_boundaries* bdry
     // the stages: Q^n stays in q_in; intermediate states rotate through the
     // two scratch states; the last stage writes q_out
     const int strips = (nx + kStripOut - 1) / kStripOut;
-    const int span = default_span_rows(strips, ny, 1);
+    const int nspans = march_spans(0, strips, ny, 1);
     const int ns = num_stages(sch.stepper);
     const double* in = q_in;
     for (int s = 0; s < ns && e == hipSuccess; ++s) {
@@ -140,7 +140,7 @@ int mh_step_noalloc(const mh_grid_size* sz, double dx, const mh_boundaries* bdry
         FormF64::Ptrs p{FormF64::Planes{const_cast<double*>(in)}, FormF64::Planes{const_cast<double*>(q_in)},
                         FormF64::Planes{out}, &clock->dt, dx, gamma, amax_slots(clock), nullptr};
         if (num_dim == 2) {
-            StageGrid g{sz->xc, sz->yc, sz->ng, nx, ny, strips, span, kind, 0};
+            StageGrid g{sz->xc, sz->yc, sz->ng, nx, ny, strips, nspans, kind, 0};
             e = launch_stage2d_f64(sch, g, p, st);
         } else {
             e = launch_stage1d_f64(sch, kind, 0, sz->xc, sz->ng, p, st);

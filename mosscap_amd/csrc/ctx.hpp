@@ -21,14 +21,19 @@ inline int stage_kind(TimeStepScheme ts, int stage) {
 
 inline bool valid_bc(int32_t b) { return b >= 0 && b <= 6; }
 
-// Default rows per wavefront of the 2-D march: one round of wavefronts, the
-// shortest span (>= 8 rows; the y halo costs 2R+2 rows per span) whose
-// wavefronts all fit 1024 SIMDs at 4 waves each, so none waits for a second
-// round; 32 rows when even that needs several rounds.
-inline int default_span_rows(int strips, int ny, int nz) {
-    for (int h = 8; h <= 32; ++h)
-        if ((int64_t)strips * ((ny + h - 1) / h) * nz <= 4096) return h;
-    return 32;
+// Spans per plane of the 2-D march (each wavefront marches one span of a
+// strip; the ny rows are split into spans whose heights differ by at most one
+// row). rows > 0: spans of about that many rows. rows <= 0, the default
+// layout: the most spans whose wavefronts all fit 1024 SIMDs at 4 waves each
+// (every SIMD then holds 4 wavefronts and none waits for a second round) with
+// at least 8 rows each (the y halo costs 2R+2 rows per span), and spans of at
+// most 32 rows when even that needs several rounds.
+inline int march_spans(int rows, int strips, int ny, int nz) {
+    if (rows > 0) return (ny + rows - 1) / rows;
+    const int64_t fill = 4096 / ((int64_t)strips * nz);
+    const int64_t n = fill < ny / 8 ? fill : ny / 8;
+    const int64_t floor32 = (ny + 31) / 32;
+    return (int)(n > floor32 ? n : (floor32 > 0 ? floor32 : 1));
 }
 
 // Message returned by mh_last_error(NULL) (functions without a context).
@@ -86,7 +91,7 @@ struct mh_ctx {
     // changes the state or the slots otherwise), so the next mh_run needs no
     // separate CFL pass.
     bool amax_valid = false;
-    int span_h = 16;
+    int nspans = 1;   // spans per plane of the 2-D march (march_spans)
     int strips = 0;
     std::string err;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
@@ -272,7 +277,7 @@ struct mh_ctx {
 
     // 3-D march grid (the interior fold of the CFL max with `reduce`)
     StageGrid grid3(int kind, int reduce) const {
-        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
         g.zc = sz.zc;
         g.nz = sz.zc - 2 * sz.ng;
         set_virt(g);
@@ -287,7 +292,7 @@ struct mh_ctx {
                              amax_slots(clock), floor_hits, opt.rho_floor, opt.p_floor, frx32, fry32,
                              gravity, grav[0], grav[1]};
             if (nd == 2) {
-                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
                 set_virt(g);
                 e = launch_stage2d_f32c(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
@@ -301,7 +306,7 @@ struct mh_ctx {
             FormF64::Ptrs p{p64(in), p64(qn_k), p64(out), &clock->dt, dx, gamma, amax_slots(clock), nullptr,
                             frx64, fry64, gravity, grav[0], grav[1]};
             if (nd == 2) {
-                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, reduce};
+                StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
                 set_virt(g);
                 e = launch_stage2d_f64(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
@@ -319,7 +324,7 @@ struct mh_ctx {
     hipError_t stage_ptr(const double* in, const double* qn_p, double* out, int kind) {
         FormF64::Ptrs p{FormF64::Planes{(double*)in}, FormF64::Planes{(double*)qn_p}, FormF64::Planes{out}, &clock->dt,
                         dx, gamma, amax_slots(clock), nullptr, nullptr, nullptr, gravity, grav[0], grav[1]};
-        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, span_h, kind, 0};
+        StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, 0};
         hipError_t e = nd == 2 ? launch_stage2d_f64(sch, g, p, stream)
                                : launch_stage1d_f64(sch, kind, 0, sz.xc, sz.ng, p, stream);
         if (e != hipSuccess) return e;

@@ -50,7 +50,7 @@ enum StageKind : int32_t {
 
 struct StageGrid {
     int32_t xc, yc, ng, nx, ny;
-    int32_t strips, span_h;
+    int32_t strips, nspans;  // 62-column strips, spans of rows per plane
     int32_t kind;        // StageKind
     int32_t reduce_dt;   // fold the new state into the CFL max
     int32_t zc = 1, nz = 1;  // 3-D: planes (ghosts included) and interior planes
@@ -89,13 +89,14 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
     const int lane = threadIdx.x;
     const uint32_t bb = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t per_plane = (uint32_t)g.strips * (uint32_t)((g.ny + g.span_h - 1) / g.span_h);
+    const uint32_t per_plane = (uint32_t)g.strips * (uint32_t)g.nspans;
     const uint32_t b = bb % per_plane;
     const int strip = (int)(b % (uint32_t)g.strips);
     const int span = (int)(b / (uint32_t)g.strips);
     const int i0 = g.ng + strip * kStripOut;
-    const int j0 = g.ng + span * g.span_h;
-    const int j1 = min(j0 + g.span_h, g.ng + g.ny);
+    // rows of this span: the ny rows split into nspans spans of equal height (+-1)
+    const int j0 = g.ng + (int)(((int64_t)span * g.ny) / g.nspans);
+    const int j1 = g.ng + (int)(((int64_t)(span + 1) * g.ny) / g.nspans);
     const int c = i0 - 1 + lane;
     // ghosts read from their sources (StageGrid::virt; compiled for the forms that use it)
     constexpr bool kVirt = Form::kVirtualGhosts;

@@ -96,7 +96,7 @@ auto pick_scheme(const Schemes& s, Pick pick) {
 template <class Form>
 hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typename Form::Ptrs& p, hipStream_t st) {
     if (g.nx < 1 || g.ny < 1 || g.ng < min_ghost_cells(s.recon) || g.xc != g.nx + 2 * g.ng ||
-        g.yc != g.ny + 2 * g.ng || g.span_h < 1 || g.strips != (g.nx + kStripOut - 1) / kStripOut) {
+        g.yc != g.ny + 2 * g.ng || g.nspans < 1 || g.nspans > g.ny || g.strips != (g.nx + kStripOut - 1) / kStripOut) {
         return hipErrorInvalidValue;
     }
     // the plain instance (buffer-resource addressing, no source, tracer
@@ -110,8 +110,7 @@ hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typen
                       : SolverPick<Form, R, SL>::template two<false>(s.solver);
     };
     Stage2dFn<Form> fn = pick_scheme<Form>(s, pick);
-    const int spans = (g.ny + g.span_h - 1) / g.span_h;
-    hipLaunchKernelGGL(fn, dim3((unsigned)(g.strips * spans)), dim3(kWave), 0, st, g, p);
+    hipLaunchKernelGGL(fn, dim3((unsigned)(g.strips * g.nspans)), dim3(kWave), 0, st, g, p);
     return hipGetLastError();
 }
 
@@ -145,7 +144,7 @@ template <class Form>
 hipError_t launch_stage3d_impl(const Schemes& s, const StageGrid& g, const typename Form::Ptrs& p,
                                typename Form::real* dz, hipStream_t st) {
     if (g.nx < 1 || g.ny < 1 || g.nz < 1 || g.ng < min_ghost_cells(s.recon) || g.xc != g.nx + 2 * g.ng ||
-        g.yc != g.ny + 2 * g.ng || g.zc != g.nz + 2 * g.ng || g.span_h < 1 ||
+        g.yc != g.ny + 2 * g.ng || g.zc != g.nz + 2 * g.ng || g.nspans < 1 || g.nspans > g.ny ||
         g.strips != (g.nx + kStripOut - 1) / kStripOut || !dz) {
         return hipErrorInvalidValue;
     }
@@ -184,9 +183,8 @@ hipError_t launch_stage3d_impl(const Schemes& s, const StageGrid& g, const typen
                        0, st, zg, p, dz);
     typename Form::Ptrs pm = p;
     pm.dz3 = dz;
-    const int spans = (g.ny + g.span_h - 1) / g.span_h;
-    hipLaunchKernelGGL(pick_scheme<Form>(s, mpick), dim3((unsigned)(g.strips * spans * g.nz)), dim3(kWave), 0, st, g,
-                       pm);
+    hipLaunchKernelGGL(pick_scheme<Form>(s, mpick), dim3((unsigned)(g.strips * g.nspans * g.nz)), dim3(kWave), 0, st,
+                       g, pm);
     return hipGetLastError();
 }
 #endif
