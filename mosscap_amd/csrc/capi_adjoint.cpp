@@ -87,6 +87,11 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
         const int rc = mh_time_step(ctx, dt);
         if (rc != MH_OK) return rc;
     }
+    // the recorded step sizes on the device, for the reverse sweep's recomputation
+    e = hipMalloc(&ctx->adj_dts_dev, (size_t)nsteps * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(ctx->adj_dts_dev, ctx->adj_dts.data(), (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_forward");
     if (dts_out)
         for (int64_t n = 0; n < nsteps; ++n) dts_out[n] = ctx->adj_dts[n];
     return MH_OK;
@@ -139,14 +144,13 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
         // recompute the segment: seg[(n-n0)*ns + s] holds the input of stage s of step n
         e = hipMemcpyAsync(seg[0].p, ctx->ckpt[sgi], nv * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
         for (int64_t n = n0; n < n1 && e == hipSuccess; ++n) {
-            e = launch_clock_set_dt(ctx->clock, ctx->adj_dts[n], ctx->stream);
             const double* qn = seg[(n - n0) * ns].p;
             for (int s = 0; s < ns && e == hipSuccess; ++s) {
                 const double* in = seg[(n - n0) * ns + s].p;
                 const bool last = (s == ns - 1);
                 if (last && n + 1 == n1) break;  // the segment's end state is not needed
                 double* out = last ? seg[(n + 1 - n0) * ns].p : seg[(n - n0) * ns + s + 1].p;
-                e = ctx->stage_ptr(in, qn, out, stage_kind(ctx->sch.stepper, s));
+                e = ctx->stage_ptr(in, qn, out, stage_kind(ctx->sch.stepper, s), ctx->adj_dts_dev + n);
             }
         }
         // reverse the segment's steps

@@ -321,8 +321,9 @@ struct mh_ctx {
     }
 
     // FP64 stage between arbitrary device buffers (adjoint recomputation).
-    hipError_t stage_ptr(const double* in, const double* qn_p, double* out, int kind) {
-        FormF64::Ptrs p{FormF64::Planes{(double*)in}, FormF64::Planes{(double*)qn_p}, FormF64::Planes{out}, &clock->dt,
+    // dt_dev: the step's dt in device memory
+    hipError_t stage_ptr(const double* in, const double* qn_p, double* out, int kind, const double* dt_dev) {
+        FormF64::Ptrs p{FormF64::Planes{(double*)in}, FormF64::Planes{(double*)qn_p}, FormF64::Planes{out}, dt_dev,
                         dx, gamma, amax_slots(clock), nullptr, nullptr, nullptr, gravity, grav[0], grav[1]};
         StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, 0};
         hipError_t e = nd == 2 ? launch_stage2d_f64(sch, g, p, stream)
@@ -334,6 +335,7 @@ struct mh_ctx {
     // adjoint records (mh_adjoint_forward / mh_adjoint_backward)
     std::vector<double*> ckpt;
     std::vector<double> adj_dts;
+    double* adj_dts_dev = nullptr;  // the same step sizes on the device (the recomputation reads them)
     int adj_every = 0;
     double adj_bwd_ms = 0.0;  // device time of the last reverse sweep
     // reverse-sweep scratch kept between mh_adjoint_backward calls (nvals
@@ -364,6 +366,8 @@ struct mh_ctx {
             if (p) hipFree(p);
         ckpt.clear();
         adj_dts.clear();
+        if (adj_dts_dev) hipFree(adj_dts_dev);
+        adj_dts_dev = nullptr;
     }
 
     // All stages of one step starting from `cur`; returns the new cur.
