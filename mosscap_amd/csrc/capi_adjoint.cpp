@@ -121,6 +121,9 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     g.gravity = ctx->gravity;
     g.gx = ctx->grav[0];
     g.gy = ctx->grav[1];
+    g.virt = ctx->virtual_ghosts() ? 1 : 0;  // the recomputed stage inputs keep unfilled ghosts
+    g.xper = ctx->bdry.xs == (int32_t)BoundaryType::Periodic;
+    g.yper = ctx->bdry.ys == (int32_t)BoundaryType::Periodic;
     // scratch, allocated by the first reverse sweep and kept by the context:
     // the x sweep's primitive cotangents, three cotangent states, the stage
     // states of one segment ((every) steps x ns stages) and pinned host

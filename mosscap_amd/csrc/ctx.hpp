@@ -259,8 +259,8 @@ struct mh_ctx {
     // boundary). A stage reads the values the reference's fill would have
     // written (Boundaries.hpp), so the results are the same bits; leaving the
     // ghosts out of the CFL max changes nothing (they copy interior values).
-    // mh_stage still fills after every stage, and the adjoint's
-    // recomputation fills as the reference does.
+    // mh_stage still fills after every stage. The adjoint's recomputation
+    // skips the fills too; its reverse sweep reads ghosts from their sources.
     bool virtual_ghosts() const {
         const auto copy_bc = [](int32_t b) {
             return b == (int32_t)BoundaryType::Periodic || b == (int32_t)BoundaryType::ZeroGrad;
@@ -321,14 +321,17 @@ struct mh_ctx {
     }
 
     // FP64 stage between arbitrary device buffers (adjoint recomputation).
-    // dt_dev: the step's dt in device memory
+    // dt_dev: the step's dt in device memory. With virtual_ghosts() the
+    // input's ghosts are read from their sources and the output's are left
+    // unfilled (the reverse sweep reads them the same way).
     hipError_t stage_ptr(const double* in, const double* qn_p, double* out, int kind, const double* dt_dev) {
         FormF64::Ptrs p{FormF64::Planes{(double*)in}, FormF64::Planes{(double*)qn_p}, FormF64::Planes{out}, dt_dev,
                         dx, gamma, amax_slots(clock), nullptr, nullptr, nullptr, gravity, grav[0], grav[1]};
         StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, 0};
+        set_virt(g);
         hipError_t e = nd == 2 ? launch_stage2d_f64(sch, g, p, stream)
                                : launch_stage1d_f64(sch, kind, 0, sz.xc, sz.ng, p, stream);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess || virtual_ghosts()) return e;
         return launch_ghost_f64(nd, ghost_grid(0), FormF64::Planes{(double*)in}, FormF64::Planes{out}, nullptr, stream);
     }
 

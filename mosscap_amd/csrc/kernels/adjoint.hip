@@ -89,10 +89,12 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
     const bool cell_in = i >= g.ng && i < g.ng + g.nx && j >= g.ng && j < g.ng + g.ny;
     const int nc = g.ncells;
 
+    const int qcell = g.virt ? copy_ghost_source(j, g.ny, g.ng, g.yper) * g.xc + copy_ghost_source(i, g.nx, g.ng, g.xper)
+                             : cell;
     double qq[4], mu[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        qq[v] = a.q[v * nc + cell];
+        qq[v] = a.q[v * nc + qcell];
         mu[v] = cell_in ? a.mu[v * nc + cell] : 0.0;
     }
     {

@@ -15,11 +15,14 @@ struct AdjGrid {
     double dt;
     int32_t gravity;  // a gravity source term is registered
     double gx, gy;
+    // the stage inputs' ghosts are not filled: ghost cells are read from
+    // their sources (every face periodic or zerograd; xper/yper: periodic)
+    int32_t virt, xper, yper;
 };
 
 // One reverse stage of the stage output a_qn*Q^n + a_qs*Q_s + a_l*L(Q_s).
 struct AdjStage {
-    const double* q;   // Q_s, ghost-filled
+    const double* q;   // Q_s, ghost-filled unless AdjGrid::virt
     const double* mu;  // cotangent of the stage output (interior; ghost entries zero)
     double* lamx;      // scratch [4][cells]
     double* nxt;       // out: a_qs*mu + a_l*J_L(Q_s)^T mu (interior; ghosts folded and zero)
