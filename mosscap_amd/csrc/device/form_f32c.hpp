@@ -87,6 +87,7 @@ struct FormF32c {
     // 2-D grids with copy-type faces read ghost rows and columns from their
     // sources instead of filling them after every stage (StageGrid::virt)
     static constexpr bool kVirtualGhosts = true;
+    static constexpr bool kHasFloors = true;  // positivity floors (floors_on)
     // the instance without buffer resources loads Q^(s) and Q^n for the
     // combine at the top of the row (their latency hides behind the sweeps)
     static constexpr bool kEarlyOperands = true;

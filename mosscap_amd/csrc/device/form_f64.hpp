@@ -40,6 +40,7 @@ struct FormF64 {
     // grids with copy-type faces read ghost rows and columns from their
     // sources instead of filling them after every stage (StageGrid::virt)
     static constexpr bool kVirtualGhosts = true;
+    static constexpr bool kHasFloors = false;  // positivity floors (floors_on)
     // the instance without buffer resources loads Q^(s) and Q^n for the
     // combine just before it (fewer registers live across the sweeps)
     static constexpr bool kEarlyOperands = false;

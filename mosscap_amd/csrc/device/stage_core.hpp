@@ -48,6 +48,12 @@ enum StageKind : int32_t {
     kSsp4Third = 5,    // ((2/3)*Qn + (1/3)*Q) + (1/6)*(dq + src)   SSPRK(4,3) stage 2
 };
 
+// Instances of the fused stage. Plain: buffer-resource addressing, operands
+// staged through LDS, no floors, gravity source or tracer stores. Floors: the
+// plain instance with the compensated form's positivity floors. Extras:
+// gravity source, tracers' density-flux stores, floors and 64-bit addressing.
+enum StageInst : int { kStagePlain = 0, kStageFloors = 1, kStageExtras = 2 };
+
 struct StageGrid {
     int32_t xc, yc, ng, nx, ny;
     int32_t strips, nspans;  // 62-column strips, spans of rows per plane
@@ -66,12 +72,12 @@ struct StageGrid {
 // interior z-plane per (strip, span, plane) workgroup; the z flux difference
 // of each cell comes from the z sweep (p.dz3) and is added after the y one,
 // as integrate_flux adds Fz after Fy (TimeStepping.cpp:22-30).
-// Extras = false compiles out the gravity source, the tracers' density-flux
-// stores and the positivity floors (their addresses, operands and branches
+// Inst selects what is compiled in (their addresses, operands and branches
 // cost registers the plain hydro stage needs); the launcher picks the
 // instance from the pointers and options it is given.
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int ND = 2, bool Extras = true>
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int ND = 2, int Inst = kStageExtras>
 __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Ptrs& p) {
+    constexpr bool Extras = Inst == kStageExtras;
     using T = typename Form::real;
     using Raw = typename Form::raw;
     constexpr int NV = 2 + ND;  // rho, m_x, m_y[, m_z], E
@@ -137,9 +143,8 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     // for the wave-uniform row/plane/variable part, fixed per-lane vector
     // offsets (the launcher keeps every byte offset below 2^31).
     constexpr bool kBuf = !Extras;
-    // positivity floors (the compensated form's option) live in the other
-    // instance too; the launchers pick it when they are on
-    constexpr bool kFloors = Extras;
+    // positivity floors (the compensated form's option)
+    constexpr bool kFloors = Inst != kStagePlain;
     const auto bqs = Form::buf(p.qs);
     const auto bqn = Form::buf(p.qn);
     const auto bout = Form::buf(p.out);

@@ -17,11 +17,11 @@
 
 namespace mh {
 
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, bool Extras>
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int Inst>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(Extras ? MH_STAGE_WAVES_EXTRAS : MH_STAGE_WAVES_PER_EU)))
+__attribute__((amdgpu_waves_per_eu(Inst == kStageExtras ? MH_STAGE_WAVES_EXTRAS : MH_STAGE_WAVES_PER_EU)))
 void stage2d_kernel(StageGrid g, typename Form::Ptrs p) {
-    stage2d_march<Form, R, SL, RS, 2, Extras>(g, p);
+    stage2d_march<Form, R, SL, RS, 2, Inst>(g, p);
 }
 
 struct Stage1dGrid {
@@ -56,12 +56,12 @@ using Stage1dFn = void (*)(Stage1dGrid, typename Form::Ptrs);
 
 template <class Form, Reconstruction R, SlopeLimiter SL>
 struct SolverPick {
-    template <bool Extras>
+    template <int Inst>
     static Stage2dFn<Form> two(RiemannSolver rs) {
         switch (rs) {
-            case RiemannSolver::Rusanov: return stage2d_kernel<Form, R, SL, RiemannSolver::Rusanov, Extras>;
-            case RiemannSolver::Hll: return stage2d_kernel<Form, R, SL, RiemannSolver::Hll, Extras>;
-            default: return stage2d_kernel<Form, R, SL, RiemannSolver::Hllc, Extras>;
+            case RiemannSolver::Rusanov: return stage2d_kernel<Form, R, SL, RiemannSolver::Rusanov, Inst>;
+            case RiemannSolver::Hll: return stage2d_kernel<Form, R, SL, RiemannSolver::Hll, Inst>;
+            default: return stage2d_kernel<Form, R, SL, RiemannSolver::Hllc, Inst>;
         }
     }
     static Stage1dFn<Form> one(RiemannSolver rs) {
@@ -100,14 +100,19 @@ hipError_t launch_stage2d_impl(const Schemes& s, const StageGrid& g, const typen
         return hipErrorInvalidValue;
     }
     // the plain instance (buffer-resource addressing, no source, tracer
-    // stores or floors) unless a gravity source, tracer flux stores or
-    // positivity floors are asked for, or the state's byte offsets do not fit
-    // the buffer offsets' 31 bits
+    // stores or floors), with the floors compiled in when they are on (forms
+    // that have them), and the general instance for a gravity source or
+    // tracer flux stores or when the state's byte offsets do not fit the
+    // buffer offsets' 31 bits
     const bool big = (int64_t)4 * g.xc * g.yc * Form::kElem >= ((int64_t)1 << 31);
-    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr || Form::floors_on(p) || big;
-    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() {
-        return extras ? SolverPick<Form, R, SL>::template two<true>(s.solver)
-                      : SolverPick<Form, R, SL>::template two<false>(s.solver);
+    const bool extras = p.gravity != 0 || p.frx != nullptr || p.fry != nullptr || big;
+    const bool floors = Form::floors_on(p);
+    auto pick = [&]<Reconstruction R, SlopeLimiter SL>() -> Stage2dFn<Form> {
+        if (extras) return SolverPick<Form, R, SL>::template two<kStageExtras>(s.solver);
+        if constexpr (Form::kHasFloors) {
+            if (floors) return SolverPick<Form, R, SL>::template two<kStageFloors>(s.solver);
+        }
+        return SolverPick<Form, R, SL>::template two<kStagePlain>(s.solver);
     };
     Stage2dFn<Form> fn = pick_scheme<Form>(s, pick);
     hipLaunchKernelGGL(fn, dim3((unsigned)(g.strips * g.nspans)), dim3(kWave), 0, st, g, p);
@@ -132,7 +137,7 @@ template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, bool 
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(Extras ? MH_STAGE_WAVES_EXTRAS : MH_STAGE_WAVES_PER_EU)))
 void stage3d_kernel(StageGrid g, typename Form::Ptrs p) {
-    stage2d_march<Form, R, SL, RS, 3, Extras>(g, p);
+    stage2d_march<Form, R, SL, RS, 3, Extras ? kStageExtras : kStagePlain>(g, p);
 }
 
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
