@@ -4,9 +4,11 @@ the torch.autograd.Function.
 * the hand-written VJPs equal the dual-number Jacobians of the same functors;
 * on a small periodic grid the gradient equals torch autograd of an independent
   float64 restatement of the step (tests/torch_hydro.py), rtol 1e-8;
-* on the full workload (KH 512^2, 200 steps, checkpoint every 20) directional
-  derivatives agree with FP64 central finite differences (eps = 1e-3) along
-  16 seeded directions to 1e-4 (relative to the largest |FD|).
+* on the full workload (KH 512^2, 200 steps, checkpoint every 20), along 16
+  seeded directions, directional derivatives agree with the exact
+  tangent-linear run of the same FP64 map to 1e-10, and with FP64 central
+  finite differences to 1e-5 at eps = 1e-5 and 1e-3 at eps = 1e-3 (relative to
+  the largest |FD|; the larger eps crosses kinks of the map, see below).
 """
 import time
 
@@ -111,8 +113,9 @@ def test_full_workload_fd_agreement(solver):
     Central differences at eps = 1e-3 carry an O(eps) error of their own: the discrete
     map has kinks (|beta0 - beta2| in WENO5-Z, the HLLC and min/max switches) that a
     perturbation of that size crosses in some cells. The deviation from the adjoint
-    shrinks with eps (1e-3: <= 5e-4 of the largest |FD|; 1e-5: ~1e-7), which is the
-    check that the adjoint is the derivative of the map it differentiates."""
+    shrinks with eps (1e-3: <= 5e-4 of the largest |FD|; 1e-5: <= 3e-6), which is the
+    check that the adjoint is the derivative of the map it differentiates; the exact check
+    is test_full_workload_tangent_agreement."""
     prob = KHAdjointProblem(n=512, nsteps=200, checkpoint_every=20, riemann_solver=solver)
     n = prob.cfg.nx
     control = 0.01 * (2 * problems.uniform01(17, np.arange(n * n)).reshape(n, n) - 1)
