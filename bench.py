@@ -246,13 +246,14 @@ def trace_summary(trace_csv):
     return 0
 
 
-def measured_traffic(kernel_prefix):
-    """Average HBM bytes per launch of the matching kernel from the committed PMC summary."""
+def measured_traffic(kernel_prefix, suffixes=("",)):
+    """Average HBM bytes per launch of the matching kernel (name = prefix + one of the suffixes, then
+    the argument list) from the committed PMC summary."""
     try:
         data = json.loads(TRAFFIC_JSON.read_text())
     except (OSError, ValueError):
         return None
-    vals = [r for n, r in data["kernels"].items() if n.startswith(kernel_prefix)]
+    vals = [r for n, r in data["kernels"].items() if any(n.startswith(kernel_prefix + s) for s in suffixes)]
     if not vals:
         return None
     return sum(v["traffic"] for v in vals) / len(vals)
@@ -400,6 +401,10 @@ def main(argv=None):
     ap.add_argument("--workload", choices=["kh", "batch", "sedov", "blast3d"], default="kh",
                     help="kh: the headline line; batch / sedov: the other workloads; blast3d: the 3-D path")
     ap.add_argument("--n3", type=int, default=256, help="cells per side of the 3-D blast")
+    ap.add_argument("--recon", default="weno5z", choices=["fog", "muscl", "ppm", "weno5z"],
+                    help="KH reconstruction (counter passes; the headline is weno5z)")
+    ap.add_argument("--solver", default="hllc", choices=["rusanov", "hll", "hllc"],
+                    help="KH Riemann solver (counter passes; the headline is hllc)")
     ap.add_argument("--batch-steps", type=int, default=500)
     a = ap.parse_args(argv)
     if a.workload == "batch":
@@ -425,6 +430,11 @@ def main(argv=None):
     from mosscap_amd import problems
 
     cfg = problems.kh_config(a.n)
+    if (a.recon, a.solver) != ("weno5z", "hllc"):  # counter passes over other schemes; not the headline
+        import dataclasses
+        cfg = dataclasses.replace(cfg, reconstruction=a.recon, riemann_solver=a.solver)
+    scheme = ("WENO5-Z + HLLC" if (a.recon, a.solver) == ("weno5z", "hllc")
+              else f"{cfg.reconstruction} + {cfg.riemann_solver}")
     q0 = problems.kh_band(cfg)
     cells = cfg.nx * cfg.ny
 
@@ -461,7 +471,7 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f32c",
         "data": "synthetic (seeded KH shear band, generated on the host)",
-        "config": {"workload": f"KH {cfg.nx}x{cfg.ny} periodic, ng=3, WENO5-Z + HLLC + SSPRK3, CFL 0.4, "
+        "config": {"workload": f"KH {cfg.nx}x{cfg.ny} periodic, ng=3, {scheme} + SSPRK3, CFL 0.4, "
                                "hipGraph-replayed steps",
                    "form": "fp32c: FP32 recon/Riemann, hi+lo FP32 state, FP64 pressure recovery and dt"},
         "fp64": {"value": round(v64, 2), "unit": "Mcell-updates/s", "ms_per_step": round(el64 * 1e3 / a.steps, 4),
@@ -473,9 +483,11 @@ def main(argv=None):
         "adjoint": adj,
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+                     # the plain instance (StageInst 0; named "false" before there were three)
                      "traffic": measured_traffic("void mh::stage2d_kernel<mh::FormF32c, (mh::Reconstruction)3, "
-                                                 "(mh::SlopeLimiter)1, (mh::RiemannSolver)2, false>"),
-                     "kernel": "stage2d_kernel<FormF32c, Weno5Z, MC, HLLC, false>",
+                                                 "(mh::SlopeLimiter)1, (mh::RiemannSolver)2, ",
+                                                 ("0>(", "false>(")),
+                     "kernel": "stage2d_kernel<FormF32c, Weno5Z, MC, HLLC, 0 (plain instance)>",
                      "avg_launch_ms": round(avg32, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "autotune": {"fp32c": tune32, "fp64": tune64},
         "kernel_split_ms_per_step": {

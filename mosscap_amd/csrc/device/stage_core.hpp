@@ -38,6 +38,12 @@ namespace mh {
 
 constexpr int kStripOut = 62;  // output columns per wavefront
 
+// Stencil rows of Q^(s) in flight per wavefront of the plain march (1 or 2);
+// each translation unit of stage kernels may set its own.
+#ifndef MH_STAGE_LEAD_ROWS
+#define MH_STAGE_LEAD_ROWS 1
+#endif
+
 // Stage update kinds: the lambdas of TimeStepping.cpp, src = S*dt with S = 0.
 enum StageKind : int32_t {
     kAddFull = 0,      // Q + (dq + src)                       RK2/SSPRK3 stage 0
@@ -225,16 +231,28 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 #pragma unroll
         for (int v = 0; v < NV; ++v) q[v] = load_qs(c_ld, v * plane + base);
     };
-    // Plain instance: the next stencil row of Q^(s) goes straight into LDS
-    // too (issued at the top of a row, complete by its combine, which waits
-    // for every vector-memory load), so no registers hold it across a row.
-    __shared__ uint32_t lead[kDmaOps ? NV : 1][2][kWave];  // [var][word][lane]
+    // Plain instance: the stencil rows of Q^(s) go straight into LDS too, so
+    // no registers hold them across a row. With kLeadRows = 1 a row is issued
+    // at the top of the iteration before the one that reads it and is
+    // complete by that iteration's combine, which waits for every load. With
+    // kLeadRows = 2 (MH_STAGE_LEAD_ROWS) it is issued two iterations ahead
+    // into one of two slots (row r in slot r & 1), and the combine waits for
+    // every load but the lead row issued at its own top (loads complete in
+    // order), so a lead row has two rows of sweeps to arrive. The 3-D FP64
+    // march takes 2 (fewer spilled registers, 21 % faster); the others 1 (no
+    // gain, and the 3-D FP32C march loses a wave per SIMD to the LDS).
+    constexpr int kLeadRows = MH_STAGE_LEAD_ROWS;
+    static_assert(kLeadRows == 1 || kLeadRows == 2, "lead rows in flight");
+    __shared__ uint32_t lead[kDmaOps ? kLeadRows : 1][kDmaOps ? NV : 1][2][kWave];  // [slot][var][word][lane]
+    constexpr int kLeadOps = 2 * NV;  // vector-memory operations of one issue_lead
     auto issue_lead = [&](int r) {
         if constexpr (kDmaOps) {
             const int base = pk + src_row(r) * xc;
+            const int sl = kLeadRows == 2 ? (r & 1) : 0;
 #pragma unroll
             for (int v = 0; v < NV; ++v)
-                Form::dma(bqs, &lead[v][0][0], &lead[v][1][0], (uint32_t)c_ld * ES, (uint32_t)(v * plane + base));
+                Form::dma(bqs, &lead[sl][v][0][0], &lead[sl][v][1][0], (uint32_t)c_ld * ES,
+                          (uint32_t)(v * plane + base));
         }
     };
     auto recon_y = [&](T (&wl)[NV], T (&wr)[NV]) {
@@ -270,6 +288,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         if constexpr (kDmaOps) {
             if (j0 < j1) {
                 issue_lead(j0 + 1 + ORD);
+                if (kLeadRows == 2 && j0 + 1 < j1) issue_lead(j0 + 2 + ORD);
                 issue_operands(j0);
                 wait_vm_loads();  // the first row reads both without a combine's wait before it
             }
@@ -303,13 +322,14 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         if constexpr (kDmaOps) {
             wave_lds_sync();
             Raw ql[NV];
+            const int sl = kLeadRows == 2 ? ((j + 1 + ORD) & 1) : 0;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ql[v] = Form::from_words(lead[v][0][lane], lead[v][1][lane]);
+            for (int v = 0; v < NV; ++v) ql[v] = Form::from_words(lead[sl][v][0][lane], lead[sl][v][1][lane]);
             commit_row(j + 1 + ORD, ql);
-            if (j + 1 < j1) {
-                wait_lds_ops();  // the lead words are read before the next row's overwrite them
+            if (j + kLeadRows < j1) {
+                wait_lds_ops();  // the slot's words are read before the next row for it overwrites them
                 wave_lds_sync();
-                issue_lead(j + 2 + ORD);
+                issue_lead(j + 1 + kLeadRows + ORD);
             }
         } else {
             commit_row(j + 1 + ORD, qa);
@@ -358,7 +378,10 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         for (int v = 0; v < NV; ++v) Fxp[v] = lane_from_right(Fx[v]);
 
         if constexpr (kDmaOps) {  // this row's operands, loaded into LDS during the previous row
-            wait_vm_loads();
+            // every load (with two lead rows in flight: but the one issued at
+            // this row's top, if any)
+            if (kLeadRows == 2 && j + 2 < j1) wait_vm_at_most<kLeadOps>();
+            else wait_vm_loads();
             wave_lds_sync();
 #pragma unroll
             for (int v = 0; v < NV; ++v) {

@@ -93,6 +93,13 @@ __device__ inline void buf_load_lds(__amdgpu_buffer_rsrc_t r, uint32_t* lds, uin
 }
 // s_waitcnt vmcnt(0): every vector-memory operation of this wave has completed.
 __device__ inline void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt vmcnt(N): at most the N most recently issued vector-memory
+// operations of this wave are still outstanding (they complete in order).
+template <int N>
+__device__ inline void wait_vm_at_most() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
 // s_waitcnt lgkmcnt(0): every LDS operation of this wave has completed.
 __device__ inline void wait_lds_ops() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 

@@ -8,6 +8,7 @@
 #endif
 #define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F32C
 #define MH_STAGE_WAVES_EXTRAS 3
+#define MH_STAGE_WAVES_FLOORS 4
 #include "stage_inst.hpp"
 
 namespace mh {

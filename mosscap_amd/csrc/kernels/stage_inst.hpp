@@ -10,16 +10,22 @@
 #ifndef MH_STAGE_WAVES_PER_EU
 #define MH_STAGE_WAVES_PER_EU 1
 #endif
-// ... and of the instance with the gravity source and tracer stores
+// ... of the instance with the gravity source and tracer stores
 #ifndef MH_STAGE_WAVES_EXTRAS
 #define MH_STAGE_WAVES_EXTRAS MH_STAGE_WAVES_PER_EU
+#endif
+// ... and of the plain instance with the positivity floors
+#ifndef MH_STAGE_WAVES_FLOORS
+#define MH_STAGE_WAVES_FLOORS MH_STAGE_WAVES_PER_EU
 #endif
 
 namespace mh {
 
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int Inst>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(Inst == kStageExtras ? MH_STAGE_WAVES_EXTRAS : MH_STAGE_WAVES_PER_EU)))
+__attribute__((amdgpu_waves_per_eu(Inst == kStageExtras   ? MH_STAGE_WAVES_EXTRAS
+                                   : Inst == kStageFloors ? MH_STAGE_WAVES_FLOORS
+                                                          : MH_STAGE_WAVES_PER_EU)))
 void stage2d_kernel(StageGrid g, typename Form::Ptrs p) {
     stage2d_march<Form, R, SL, RS, 2, Inst>(g, p);
 }
