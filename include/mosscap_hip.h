@@ -283,8 +283,10 @@ int mh_adjoint_tangent(mh_ctx* ctx, const double* dq0, double* dq_final, double*
  * host transfers excluded. */
 int mh_adjoint_backward_ms(const mh_ctx* ctx, double* ms);
 /* ---- one step without device allocations (no counterpart in the reference) --
- * For callers that cannot allocate device memory: code under HIP graph
- * capture, tight serving loops, harnesses that forbid hipMalloc. The step of
+ * For callers that cannot allocate device memory: tight serving loops,
+ * harnesses that forbid hipMalloc, code that runs beside a graph-captured
+ * pipeline (the call itself synchronizes its stream before returning, so it
+ * cannot be captured into a graph). The step of
  * mh_compute_dt + mh_time_step (FP64 drop-in arithmetic, bit for bit the same
  * result) on a caller-owned state, with no context: q_in is Q^n on the device
  * ([var][k][j][i] FP64, ghosts filled), q_out receives Q^{n+1} (a distinct
