@@ -70,28 +70,33 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
     }
     ctx->adj_every = checkpoint_every;
     ctx->adj_dts.resize(nsteps);
+    // a failed forward leaves no record, so a reverse sweep cannot start from it
+    const auto failed = [&](int rc) {
+        ctx->free_adjoint();
+        return rc;
+    };
     for (int64_t n = 0; n < nsteps; ++n) {
         if (n % checkpoint_every == 0) {
             e = hipMemcpyAsync(ctx->ckpt[n / checkpoint_every], ctx->q64[ctx->cur], bytes, hipMemcpyDeviceToDevice,
                                ctx->stream);
-            if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_forward");
+            if (e != hipSuccess) return failed(ctx->hip_fail(e, "mh_adjoint_forward"));
         }
         double dt = 0.0;
         if (dts_in) {
             dt = dts_in[n];
         } else {
             const int rc = mh_compute_dt(ctx, max_cfl, 0.0, 1e300, 0.0, 1e300, &dt);
-            if (rc != MH_OK) return rc;
+            if (rc != MH_OK) return failed(rc);
         }
         ctx->adj_dts[n] = dt;
         const int rc = mh_time_step(ctx, dt);
-        if (rc != MH_OK) return rc;
+        if (rc != MH_OK) return failed(rc);
     }
     // the recorded step sizes on the device, for the reverse sweep's recomputation
     e = hipMalloc(&ctx->adj_dts_dev, (size_t)nsteps * sizeof(double));
     if (e == hipSuccess)
         e = hipMemcpy(ctx->adj_dts_dev, ctx->adj_dts.data(), (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return ctx->hip_fail(e, "mh_adjoint_forward");
+    if (e != hipSuccess) return failed(ctx->hip_fail(e, "mh_adjoint_forward"));
     if (dts_out)
         for (int64_t n = 0; n < nsteps; ++n) dts_out[n] = ctx->adj_dts[n];
     return MH_OK;
@@ -100,7 +105,8 @@ int mh_adjoint_forward(mh_ctx* ctx, int64_t nsteps, int32_t checkpoint_every, do
 int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_initial) {
     if (!ctx || !lambda_final || !lambda_initial) return MH_ERR_INVALID;
     const int64_t nsteps = (int64_t)ctx->adj_dts.size();
-    if (nsteps == 0 || ctx->ckpt.empty()) return ctx->fail(MH_ERR_INVALID, "run mh_adjoint_forward first");
+    if (nsteps == 0 || ctx->ckpt.empty() || !ctx->adj_dts_dev)
+        return ctx->fail(MH_ERR_INVALID, "run mh_adjoint_forward first");
     hipError_t e = hipSetDevice(ctx->device);
     ctx->amax_valid = false;  // the clock's CFL slots are reused here
     const int ns = num_stages(ctx->sch.stepper);
