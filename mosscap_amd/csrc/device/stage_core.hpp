@@ -43,6 +43,11 @@ constexpr int kStripOut = 62;  // output columns per wavefront
 #ifndef MH_STAGE_LEAD_ROWS
 #define MH_STAGE_LEAD_ROWS 1
 #endif
+// 1: the plain march keeps the raw Q^(s) of the last ORD+1 stencil rows in
+// registers for the stage combine instead of loading row j again for it.
+#ifndef MH_STAGE_QS_RING
+#define MH_STAGE_QS_RING 0
+#endif
 
 // Stage update kinds: the lambdas of TimeStepping.cpp, src = S*dt with S = 0.
 enum StageKind : int32_t {
@@ -172,15 +177,20 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     // is swept: no registers held across the sweeps (the compensated form
     // held 16) and no load latency at the combine (the FP64 form's).
     constexpr bool kDmaOps = kBuf;
-    __shared__ uint32_t opnd[kDmaOps ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][word][lane]
+    // With the Q^(s) ring (kQsRing) only Q^n is loaded for the combine.
+    constexpr bool kQsRing = kDmaOps && MH_STAGE_QS_RING != 0 && Inst == kStagePlain;
+    constexpr int kQnSlot = kQsRing ? 0 : 1;
+    __shared__ uint32_t opnd[kDmaOps && !kQsRing ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][word][lane]
     auto issue_operands = [&](int jr) {
         if constexpr (kDmaOps) {
             const int row = pk + jr * xc;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                Form::dma(bqs, &opnd[0][v][0][0], &opnd[0][v][1][0], (uint32_t)c_ld * ES, (uint32_t)(v * plane + row));
+                if (!kQsRing)
+                    Form::dma(bqs, &opnd[0][v][0][0], &opnd[0][v][1][0], (uint32_t)c_ld * ES,
+                              (uint32_t)(v * plane + row));
                 if (Form::needs_qn(g.kind))
-                    Form::dma(bqn, &opnd[1][v][0][0], &opnd[1][v][1][0], (uint32_t)c_ld * ES,
+                    Form::dma(bqn, &opnd[kQnSlot][v][0][0], &opnd[kQnSlot][v][1][0], (uint32_t)c_ld * ES,
                               (uint32_t)(v * plane + row));
             }
         }
@@ -243,12 +253,18 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     // gain, and the 3-D FP32C march loses a wave per SIMD to the LDS).
     constexpr int kLeadRows = MH_STAGE_LEAD_ROWS;
     static_assert(kLeadRows == 1 || kLeadRows == 2, "lead rows in flight");
-    __shared__ uint32_t lead[kDmaOps ? kLeadRows : 1][kDmaOps ? NV : 1][2][kWave];  // [slot][var][word][lane]
+    // two slots with two rows in flight, or with the Q^(s) ring (a row's raw
+    // words are read again at the end of the iteration that committed it)
+    constexpr int kLeadSlots = (kLeadRows == 2 || kQsRing) ? 2 : 1;
+    __shared__ uint32_t lead[kDmaOps ? kLeadSlots : 1][kDmaOps ? NV : 1][2][kWave];  // [slot][var][word][lane]
     constexpr int kLeadOps = 2 * NV;  // vector-memory operations of one issue_lead
+    auto lead_slot = [&](int r) { return kLeadSlots == 2 ? (r & 1) : 0; };
+    // raw Q^(s) of rows j .. j+ORD at the combine of row j (kQsRing)
+    Raw qsr[kQsRing ? ORD + 1 : 1][NV];
     auto issue_lead = [&](int r) {
         if constexpr (kDmaOps) {
             const int base = pk + src_row(r) * xc;
-            const int sl = kLeadRows == 2 ? (r & 1) : 0;
+            const int sl = lead_slot(r);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 Form::dma(bqs, &lead[sl][v][0][0], &lead[sl][v][1][0], (uint32_t)c_ld * ES,
@@ -270,9 +286,17 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     T yRp[NV], Gyp[NV];
     {
         Raw qb[NV];
-        for (int r = j0 - 1 - ORD; r <= j0 - 1 + ORD; ++r) {
+#pragma unroll
+        for (int k = 0; k < 2 * ORD + 1; ++k) {  // rows j0-1-ORD .. j0-1+ORD
+            const int r = j0 - 1 - ORD + k;
             fetch_row(r, qb);
             commit_row(r, qb);
+            if constexpr (kQsRing) {
+                if (k >= ORD + 1) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qsr[k - ORD - 1][v] = qb[v];
+                }
+            }
         }
         T yl[NV], yr[NV];
         recon_y(yl, yr);  // cell j0-1
@@ -280,6 +304,10 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         for (int v = 0; v < NV; ++v) yRp[v] = yr[v];
         fetch_row(j0 + ORD, qb);
         commit_row(j0 + ORD, qb);
+        if constexpr (kQsRing) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qsr[ORD][v] = qb[v];
+        }
         recon_y(yl, yr);  // cell j0
         riemann_flux<RS, 1, ND>(eos, yRp, yl, Gyp);
         if (Extras && ND == 2 && p.fry != nullptr && out_lane && j0 < j1) p.fry[j0 * xc + c] = Gyp[0];
@@ -322,7 +350,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         if constexpr (kDmaOps) {
             wave_lds_sync();
             Raw ql[NV];
-            const int sl = kLeadRows == 2 ? ((j + 1 + ORD) & 1) : 0;
+            const int sl = lead_slot(j + 1 + ORD);
 #pragma unroll
             for (int v = 0; v < NV; ++v) ql[v] = Form::from_words(lead[sl][v][0][lane], lead[sl][v][1][lane]);
             commit_row(j + 1 + ORD, ql);
@@ -385,8 +413,11 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             wave_lds_sync();
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                qs_c[v] = Form::from_words(opnd[0][v][0][lane], opnd[0][v][1][lane]);
-                qn_c[v] = Form::needs_qn(g.kind) ? Form::from_words(opnd[1][v][0][lane], opnd[1][v][1][lane]) : qs_c[v];
+                if constexpr (kQsRing) qs_c[v] = qsr[0][v];
+                else qs_c[v] = Form::from_words(opnd[0][v][0][lane], opnd[0][v][1][lane]);
+                qn_c[v] = Form::needs_qn(g.kind)
+                              ? Form::from_words(opnd[kQnSlot][v][0][lane], opnd[kQnSlot][v][1][lane])
+                              : qs_c[v];
             }
         }
         if (out_lane) {
@@ -421,6 +452,17 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             if (g.reduce_dt) {
                 const double a = Form::template signal_speed_nd<ND>((double)p.gamma, qo);
                 amax = a > amax ? a : amax;
+            }
+        }
+        if constexpr (kQsRing) {  // drop row j, take row j+1+ORD (committed at this row's top)
+            if (j + 1 < j1) {
+#pragma unroll
+                for (int k = 0; k < ORD; ++k)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qsr[k][v] = qsr[k + 1][v];
+                const int sl = lead_slot(j + 1 + ORD);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qsr[ORD][v] = Form::from_words(lead[sl][v][0][lane], lead[sl][v][1][lane]);
             }
         }
         if constexpr (kDmaOps) {

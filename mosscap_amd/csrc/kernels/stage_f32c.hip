@@ -9,6 +9,12 @@
 #define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F32C
 #define MH_STAGE_WAVES_EXTRAS 3
 #define MH_STAGE_WAVES_FLOORS 4
+// The raw Q^(s) of the stencil rows stays in registers for the combine
+// (no second load of row j).
+#ifndef MH_STAGE_QS_RING_F32C
+#define MH_STAGE_QS_RING_F32C 1
+#endif
+#define MH_STAGE_QS_RING MH_STAGE_QS_RING_F32C
 #include "stage_inst.hpp"
 
 namespace mh {

@@ -4,6 +4,12 @@
 #define MH_STAGE_WAVES_F64 1
 #endif
 #define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F64
+// The raw Q^(s) of the stencil rows stays in registers for the combine
+// (no second load of row j).
+#ifndef MH_STAGE_QS_RING_F64
+#define MH_STAGE_QS_RING_F64 1
+#endif
+#define MH_STAGE_QS_RING MH_STAGE_QS_RING_F64
 #include "stage_inst.hpp"
 
 namespace mh {
