@@ -178,7 +178,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     // held 16) and no load latency at the combine (the FP64 form's).
     constexpr bool kDmaOps = kBuf;
     // With the Q^(s) ring (kQsRing) only Q^n is loaded for the combine.
-    constexpr bool kQsRing = kDmaOps && MH_STAGE_QS_RING != 0 && Inst == kStagePlain;
+    // (the floors instance of WENO5-Z has no registers to spare for it)
+    constexpr bool kQsRing =
+        kDmaOps && MH_STAGE_QS_RING != 0 && (Inst == kStagePlain || R != Reconstruction::Weno5Z);
     constexpr int kQnSlot = kQsRing ? 0 : 1;
     __shared__ uint32_t opnd[kDmaOps && !kQsRing ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][word][lane]
     auto issue_operands = [&](int jr) {
