@@ -43,8 +43,11 @@ constexpr int kStripOut = 62;  // output columns per wavefront
 #ifndef MH_STAGE_LEAD_ROWS
 #define MH_STAGE_LEAD_ROWS 1
 #endif
-// 1: the plain march keeps the raw Q^(s) of the last ORD+1 stencil rows in
-// registers for the stage combine instead of loading row j again for it.
+// The plain march keeps the raw Q^(s) of the last ORD+1 stencil rows in
+// registers for the stage combine instead of loading row j again for it:
+// 0 never, 1 always (but not in the floors instance of WENO5-Z, which has no
+// registers to spare), 2 for WENO5-Z only (the FP64 form, whose other
+// schemes would lose a wave per SIMD to the registers).
 #ifndef MH_STAGE_QS_RING
 #define MH_STAGE_QS_RING 0
 #endif
@@ -178,9 +181,9 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
     // held 16) and no load latency at the combine (the FP64 form's).
     constexpr bool kDmaOps = kBuf;
     // With the Q^(s) ring (kQsRing) only Q^n is loaded for the combine.
-    // (the floors instance of WENO5-Z has no registers to spare for it)
     constexpr bool kQsRing =
-        kDmaOps && MH_STAGE_QS_RING != 0 && (Inst == kStagePlain || R != Reconstruction::Weno5Z);
+        kDmaOps && ((MH_STAGE_QS_RING == 1 && (Inst == kStagePlain || R != Reconstruction::Weno5Z)) ||
+                    (MH_STAGE_QS_RING == 2 && R == Reconstruction::Weno5Z));
     constexpr int kQnSlot = kQsRing ? 0 : 1;
     __shared__ uint32_t opnd[kDmaOps && !kQsRing ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][word][lane]
     auto issue_operands = [&](int jr) {

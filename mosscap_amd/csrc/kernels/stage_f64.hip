@@ -4,10 +4,11 @@
 #define MH_STAGE_WAVES_F64 1
 #endif
 #define MH_STAGE_WAVES_PER_EU MH_STAGE_WAVES_F64
-// The raw Q^(s) of the stencil rows stays in registers for the combine
-// (no second load of row j).
+// The raw Q^(s) of the stencil rows stays in registers for the combine (no
+// second load of row j) with WENO5-Z; the other schemes keep 3 waves per
+// SIMD without it (PPM + HLL: 154 VGPRs, 180 with it).
 #ifndef MH_STAGE_QS_RING_F64
-#define MH_STAGE_QS_RING_F64 1
+#define MH_STAGE_QS_RING_F64 2
 #endif
 #define MH_STAGE_QS_RING MH_STAGE_QS_RING_F64
 #include "stage_inst.hpp"
