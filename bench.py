@@ -301,8 +301,43 @@ def bench_batch(a):
         out["per_problem_launches"] = {"steps": n2, "value": round(cells * n2 / el2 / 1e6, 2),
                                        "ms_per_step": round(el2 * 1e3 / n2, 3)}
         out["persistent_over_per_problem"] = round(out["value"] / out["per_problem_launches"]["value"], 2)
+    # 1-D SSPRK3, 3 variables: 192 algorithmic bytes per cell-step (stage 0 reads Q^(s) and writes
+    # Q^(s+1), stages 1-2 also read Q^n; 8 bytes per value in both forms)
+    gbs = out["value"] * 1e6 * 192.0 / 1e9
+    out["algorithmic_GBs"] = round(gbs, 1)
+    out["roofline_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(lengths, q0, a.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def cpu_baseline_batch(lengths, q0, seconds):
+    """The oracle (FP64, no floors) on the ensemble's members one after another, as one reference
+    Simulation per problem would run them: 20 steps of each member in seeded order until about
+    `seconds` of work, in cell-updates per second."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import oracle
+    import mosscap_amd as M
+    offs = np.concatenate([[0], np.cumsum(3 * (lengths + 6))])
+    cells, members, t0 = 0, 0, time.perf_counter()
+    for p, n in enumerate(lengths):
+        cfg = M.Config(num_dim=1, nx=int(n), ng=3, bc=("zerograd", "zerograd", "wall", "wall"), max_cfl=0.4,
+                       max_time=1e30, output_delta=1e30, reconstruction="weno5z", riemann_solver="hllc",
+                       time_stepper="ssprk3")
+        sim = oracle.OracleSim(cfg)
+        sim.set_q(q0[offs[p]:offs[p + 1]].reshape(cfg.shape))
+        sim.fill_bcs()
+        sim.run(max_steps=20)
+        cells += int(n) * 20
+        members += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    return {"value": cells / el / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"20 SSPRK3 steps of each of the first {members} members ({cells // 20} cells), FP64 oracle, "
+                      f"{el:.1f} s on {threads} threads"}
 
 
 def bench_sedov(a):
@@ -336,6 +371,11 @@ def bench_sedov(a):
                                   "zerograd, t_end 0.05 (steps past t_end have dt = 0)"},
            "fp64": {"steps": s64, "t": t64, "value": round(cells * s64 / el64 / 1e6, 1)},
            "fp32c": {"steps": s32, "t": t32, "value": round(cells * s32 / el32 / 1e6, 1), "floor_hits": hits},
+           # 256 algorithmic bytes per cell-step (2-D SSPRK3, 4 variables, 8 bytes per value)
+           "algorithmic_GBs": {"fp64": round(cells * s64 / el64 * 256 / 1e9, 1),
+                               "fp32c": round(cells * s32 / el32 * 256 / 1e9, 1)},
+           "roofline_frac": {"fp64": round(cells * s64 / el64 * 256 / 1e9 / HBM_PEAK_GBS, 4),
+                             "fp32c": round(cells * s32 / el32 * 256 / 1e9 / HBM_PEAK_GBS, 4)},
            "rel_l1_fp32c_vs_fp64": [float(f"{r:.3e}") for r in _rel_l1(q32, q64, cfg.ng)]}
     print(json.dumps(out), flush=True)
     return 0
@@ -368,7 +408,9 @@ def bench_blast3d(a):
         el, tm, tune = res[form]
         v = cells * a.steps / el / 1e6
         out[name] = {"value": round(v, 1), "ms_per_step": round(el / a.steps * 1e3, 4),
-                     "algorithmic_GBs": round(v * 1e6 * 320 / 1e9, 1), "span_rows": tune["span_rows"] if tune else None,
+                     "algorithmic_GBs": round(v * 1e6 * 320 / 1e9, 1),
+                     "roofline_frac": round(v * 1e6 * 320 / 1e9 / HBM_PEAK_GBS, 4),
+                     "span_rows": tune["span_rows"] if tune else None,
                      "kernel_ms_per_step": {k: round(getattr(tm, k) / a.steps, 4) for k in ("stage_ms", "ghost_ms", "clock_ms")}}
     g = cfg.ng
     a32, a64 = states[M.FP32C][:, g:-g, g:-g, g:-g], states[M.FP64][:, g:-g, g:-g, g:-g]
