@@ -539,7 +539,7 @@ class BatchContext:
         self._c(lib().mh_batch_set_waves(self._h, waves))
 
     def autotune(self, steps=3, waves_per_cu=None):
-        cand = np.array(list(waves_per_cu or [2, 4, 6, 8, 12, 16, 24, 32]) + [0], dtype=np.int32)
+        cand = np.array(list(waves_per_cu or [4, 8, 12, 16, 20, 24, 28, 32]) + [0], dtype=np.int32)
         ms = np.zeros(len(cand))
         best = C.c_int32()
         self._c(lib().mh_batch_autotune(self._h, steps, _ptr(cand), C.byref(best), _ptr(ms)))

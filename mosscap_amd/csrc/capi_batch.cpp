@@ -40,7 +40,7 @@ struct mh_batch {
     float* qhi[3] = {nullptr, nullptr, nullptr};
     float* qlo[3] = {nullptr, nullptr, nullptr};
     double* staging = nullptr;
-    int2* items = nullptr;
+    int4* items = nullptr;
     int* d_base = nullptr;
     int* d_xc = nullptr;
     double* d_dx = nullptr;
@@ -202,7 +202,7 @@ int mh_batch_create(int32_t nprob, const int32_t* lengths, int32_t ng, double x_
     b->bdry = *bdry;
     b->gamma = gamma;
     if (opts) b->opt = *opts;
-    std::vector<int2> items;
+    std::vector<int4> items;  // (problem, first output cell, block offset, cells incl. ghosts)
     int64_t off = 0;
     for (int p = 0; p < nprob; ++p) {
         const int n = lengths[p];
@@ -215,7 +215,8 @@ int mh_batch_create(int32_t nprob, const int32_t* lengths, int32_t ng, double x_
         b->base.push_back((int)off);
         b->dx.push_back(x_dim / n);  // setup_grid, SetupModules.hpp:29-31
         off += 3LL * (n + 2 * ng);
-        for (int i0 = ng; i0 < ng + n; i0 += kStripOut) items.push_back(make_int2(p, i0));
+        for (int i0 = ng; i0 < ng + n; i0 += kStripOut)
+            items.push_back(make_int4(p, i0, b->base.back(), n + 2 * ng));
     }
     if (off >= (int64_t)1 << 31) {
         delete b;
@@ -238,7 +239,7 @@ int mh_batch_create(int32_t nprob, const int32_t* lengths, int32_t ng, double x_
         }
     }
     if (e == hipSuccess && b->fp32c()) e = hipMalloc(&b->staging, off * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&b->items, items.size() * sizeof(int2));
+    if (e == hipSuccess) e = hipMalloc(&b->items, items.size() * sizeof(int4));
     if (e == hipSuccess) e = hipMalloc(&b->d_base, nprob * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&b->d_xc, nprob * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&b->d_dx, nprob * sizeof(double));
@@ -246,7 +247,7 @@ int mh_batch_create(int32_t nprob, const int32_t* lengths, int32_t ng, double x_
     if (e == hipSuccess) e = hipMalloc(&b->d_amax, nprob * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&b->d_time, nprob * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&b->floor_hits, sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMemcpy(b->items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->items, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_base, b->base.data(), nprob * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_xc, b->xc.data(), nprob * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_dx, b->dx.data(), nprob * sizeof(double), hipMemcpyHostToDevice);
@@ -384,7 +385,7 @@ int mh_batch_set_waves(mh_batch* b, int32_t waves) {
 
 int mh_batch_autotune(mh_batch* b, int32_t steps, const int32_t* waves_per_cu, int32_t* best_waves, double* ms_out) {
     if (!b || !best_waves || steps < 1) return MH_ERR_INVALID;
-    static const int32_t kDefault[] = {2, 4, 6, 8, 12, 16, 24, 32, 0};
+    static const int32_t kDefault[] = {4, 8, 12, 16, 20, 24, 28, 32, 0};
     const int32_t* cand = waves_per_cu ? waves_per_cu : kDefault;
     hipError_t e = hipSetDevice(b->device);
     const size_t bytes = (size_t)b->nvals * (b->fp32c() ? sizeof(float) : sizeof(double));

@@ -14,7 +14,7 @@
 namespace mh {
 
 struct BatchArrays {
-    const int2* items;          // (problem, first output cell i0)
+    const int4* items;          // (problem, first output cell i0, block offset, cells incl. ghosts)
     const int* base;            // element offset of each problem's block
     const int* xc;              // cells incl. ghosts
     const double* dx;
@@ -28,26 +28,42 @@ struct BatchGrid {
     int32_t virt, per;  // ghosts read from their sources (periodic `per`, else zerograd); no fill per stage
 };
 
+// Each wave walks its items with the next item's loads issued before the
+// current item's arithmetic (the item carries the block offset and length,
+// so those loads wait for nothing but the item itself).
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
 __device__ inline void batch_stage(const BatchGrid& g, const typename Form::Ptrs& p, const BatchArrays& b) {
     using T = typename Form::real;
     constexpr int ORD = stencil_order(R);
     __shared__ T lds[3 * (kWave + 2 * ORD)];
     const int lane = threadIdx.x & 63;
+    const int virt = Form::kVirtualGhosts ? g.virt : 0;
+    const bool per = g.per != 0;
     unsigned int hits = 0;
-    for (int it = blockIdx.x; it < g.nitems; it += gridDim.x) {
-        const int2 w = b.items[it];
-        const int pr = w.x, i0 = w.y;
-        const int xc = b.xc[pr];
+    int it = blockIdx.x;
+    int4 w = it < g.nitems ? b.items[it] : make_int4(0, 0, 0, 0);
+    SegRaw<Form> cur;
+    if (it < g.nitems) seg_load<Form, R>(p, g.kind, w.z, w.w, g.ng, w.w - 2 * g.ng, w.y, virt, per, cur);
+    for (; it < g.nitems; it += gridDim.x) {
+        const int nit = it + (int)gridDim.x;
+        int4 wn = w;
+        SegRaw<Form> nxt = cur;
+        if (nit < g.nitems) {
+            wn = b.items[nit];
+            seg_load<Form, R>(p, g.kind, wn.z, wn.w, g.ng, wn.w - 2 * g.ng, wn.y, virt, per, nxt);
+        }
+        const int pr = w.x, i0 = w.y, base = w.z, xc = w.w;
         const T dt = (T)b.dt[pr];
         const T dtdx = dt / (T)b.dx[pr];
         double amax = 0.0;
-        stage1d_segment<Form, R, SL, RS>(p, g.kind, g.reduce_dt, b.base[pr], xc, g.ng, xc - 2 * g.ng, i0, dtdx,
-                                         dt, lds, amax, hits, Form::kVirtualGhosts ? g.virt : 0, g.per != 0);
+        seg_compute<Form, R, SL, RS>(p, g.kind, g.reduce_dt, base, xc, g.ng, xc - 2 * g.ng, i0, dtdx, dt, lds, amax,
+                                     hits, cur);
         if (g.reduce_dt) {
             const uint64_t m = wave_max_u64(__builtin_bit_cast(uint64_t, amax));
             if (lane == 0) atomicMax(&b.amax[pr], (unsigned long long)m);
         }
+        w = wn;
+        cur = nxt;
     }
     if (p.floor_hits != nullptr) {
         const unsigned int total = wave_sum_u32(hits);

@@ -494,16 +494,19 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
 // 1-D stage: one wavefront covers 62 output cells of a (segment of a) row;
 // lanes are cells c = i0 - 1 + t, the halo entries come from lanes [0, 2R).
-template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
-__device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, int reduce_dt, int base, int xc,
-                                       int ng, int nx, int i0, const typename Form::real dtdx,
-                                       const typename Form::real dt, typename Form::real* lds_row /*[3][64+2R]*/,
-                                       double& amax, unsigned int& floor_hits, int virt = 0, bool per = false) {
-    using T = typename Form::real;
-    using Raw = typename Form::raw;
+// seg_load issues the segment's loads (raw Q^(s), Q^n and the halo columns),
+// seg_compute does the rest, so a caller can overlap one segment's loads with
+// the previous segment's arithmetic.
+template <class Form>
+struct SegRaw {
+    typename Form::raw q[3], qn[3], qh[3];
+};
+
+template <class Form, Reconstruction R>
+__device__ inline void seg_load(const typename Form::Ptrs& p, int kind, int base, int xc, int ng, int nx, int i0,
+                                int virt, bool per, SegRaw<Form>& s) {
     constexpr int NV = 3;  // rho, mx, E
     constexpr int ORD = stencil_order(R);
-    constexpr int ROW = kWave + 2 * ORD;
     const int lane = threadIdx.x & 63;
     const int c = i0 - 1 + lane;
     // virt: ghosts are read from their sources (periodic `per`, else zerograd)
@@ -512,24 +515,39 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
     const int h_entry = lane < ORD ? lane : kWave + lane;
     const int h_col = virt ? copy_ghost_source(min(i0 - 1 - ORD + h_entry, xc - 1), nx, ng, per)
                            : min(i0 - 1 - ORD + h_entry, xc - 1);
-    const EosT<T> eos = make_eos<T>((double)p.gamma);
-
-    Raw q[NV], qn[NV], qh[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        q[v] = Form::get(p.qs, base + v * xc + c_ld);
-        qn[v] = Form::needs_qn(kind) ? Form::get(p.qn, base + v * xc + c_ld) : q[v];
-        if (ORD > 0 && halo_lane) qh[v] = Form::get(p.qs, base + v * xc + h_col);
+        s.q[v] = Form::get(p.qs, base + v * xc + c_ld);
+        s.qn[v] = Form::needs_qn(kind) ? Form::get(p.qn, base + v * xc + c_ld) : s.q[v];
+        if (ORD > 0 && halo_lane) s.qh[v] = Form::get(p.qs, base + v * xc + h_col);
     }
+}
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__device__ inline void seg_compute(const typename Form::Ptrs& p, int kind, int reduce_dt, int base, int xc, int ng,
+                                   int nx, int i0, const typename Form::real dtdx, const typename Form::real dt,
+                                   typename Form::real* lds_row /*[3][64+2R]*/, double& amax,
+                                   unsigned int& floor_hits, const SegRaw<Form>& s) {
+    using T = typename Form::real;
+    using Raw = typename Form::raw;
+    constexpr int NV = 3;  // rho, mx, E
+    constexpr int ORD = stencil_order(R);
+    constexpr int ROW = kWave + 2 * ORD;
+    const int lane = threadIdx.x & 63;
+    const int c = i0 - 1 + lane;
+    const bool halo_lane = lane < 2 * ORD;
+    const int h_entry = lane < ORD ? lane : kWave + lane;
+    const EosT<T> eos = make_eos<T>((double)p.gamma);
+
     T w[NV];
     const bool out_lane = lane >= 1 && lane <= kStripOut && c < ng + nx;
-    const bool in_hit = Form::template to_prim_nd<1>(p, q, w);
+    const bool in_hit = Form::template to_prim_nd<1>(p, s.q, w);
     wave_lds_sync();
 #pragma unroll
     for (int v = 0; v < NV; ++v) lds_row[v * ROW + lane + ORD] = w[v];
     if (ORD > 0 && halo_lane) {
         T wh[NV];
-        Form::template to_prim_nd<1>(p, qh, wh);
+        Form::template to_prim_nd<1>(p, s.qh, wh);
 #pragma unroll
         for (int v = 0; v < NV; ++v) lds_row[v * ROW + h_entry] = wh[v];
     }
@@ -537,8 +555,8 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
     T xl[NV], xr[NV], xrm[NV], Fx[NV], Fxp[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        auto s = [&](int o) -> T { return lds_row[v * ROW + lane + ORD + o]; };
-        reconstruct_stencil<R, SL, T>(s, xl[v], xr[v]);
+        auto st = [&](int o) -> T { return lds_row[v * ROW + lane + ORD + o]; };
+        reconstruct_stencil<R, SL, T>(st, xl[v], xr[v]);
     }
     Form::template floor_face<1>(p, xl);
     Form::template floor_face<1>(p, xr);
@@ -551,7 +569,7 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
         Raw qo[NV];
         T srcv[NV];
         if (p.gravity) {
-            Form::template gravity_src<1>(p, q, dt, srcv);
+            Form::template gravity_src<1>(p, s.q, dt, srcv);
         } else {
 #pragma unroll
             for (int v = 0; v < NV; ++v) srcv[v] = T(0) * dt;
@@ -560,8 +578,8 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
         for (int v = 0; v < NV; ++v) {
             T q_update = Form::kExactZeros ? T(0) + (Fx[v] - Fxp[v]) : Fx[v] - Fxp[v];
             q_update *= dtdx;
-            qo[v] = p.gravity ? Form::combine(kind, q[v], qn[v], q_update, srcv[v])
-                              : Form::combine_nosrc(kind, q[v], qn[v], q_update);
+            qo[v] = p.gravity ? Form::combine(kind, s.q[v], s.qn[v], q_update, srcv[v])
+                              : Form::combine_nosrc(kind, s.q[v], s.qn[v], q_update);
         }
         const bool st_hit = Form::template floor_state<1>(p, qo);
         floor_hits += (in_hit || st_hit) ? 1u : 0u;
@@ -572,6 +590,16 @@ __device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, i
             amax = a > amax ? a : amax;
         }
     }
+}
+
+template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
+__device__ inline void stage1d_segment(const typename Form::Ptrs& p, int kind, int reduce_dt, int base, int xc,
+                                       int ng, int nx, int i0, const typename Form::real dtdx,
+                                       const typename Form::real dt, typename Form::real* lds_row /*[3][64+2R]*/,
+                                       double& amax, unsigned int& floor_hits, int virt = 0, bool per = false) {
+    SegRaw<Form> s;
+    seg_load<Form, R>(p, kind, base, xc, ng, nx, i0, virt, per, s);
+    seg_compute<Form, R, SL, RS>(p, kind, reduce_dt, base, xc, ng, nx, i0, dtdx, dt, lds_row, amax, floor_hits, s);
 }
 
 }  // namespace mh

@@ -1,6 +1,9 @@
 // Batched 1-D ensemble, compensated FP32 form (-ffp-contract=fast; the
 // error-free transforms pin contraction off themselves).
 #define MH_STAGE_WAVES_PER_EU 1
+// 5 waves per SIMD fit without spills with the next item's loads in flight
+// (96 VGPRs); the 4096-problem ensemble is fastest at 20 waves per CU.
+#define MH_BATCH_WAVES_PER_EU 5
 #include "batch_inst.hpp"
 
 namespace mh {

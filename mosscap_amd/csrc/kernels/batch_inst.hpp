@@ -8,8 +8,15 @@
 
 namespace mh {
 
+// Occupancy the register allocator must allow for the persistent stage
+// (waves per SIMD); the autotuner picks how many of them to launch.
+#ifndef MH_BATCH_WAVES_PER_EU
+#define MH_BATCH_WAVES_PER_EU 1
+#endif
+
 template <class Form, Reconstruction R, SlopeLimiter SL, RiemannSolver RS>
-__global__ __launch_bounds__(64) void batch_stage_kernel(BatchGrid g, typename Form::Ptrs p, BatchArrays b) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MH_BATCH_WAVES_PER_EU))) void batch_stage_kernel(
+    BatchGrid g, typename Form::Ptrs p, BatchArrays b) {
     batch_stage<Form, R, SL, RS>(g, p, b);
 }
 
