@@ -457,8 +457,13 @@ int mh_run(mh_ctx* ctx, int64_t nsteps, double max_cfl, double* time, double max
                 hipGraph_t g = nullptr;
                 e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) break;
-                hipError_t ce = launch_clock_finalize(ctx->clock, 1, ctx->stream);
-                if (ce == hipSuccess) ctx->enqueue_step(k, true, ce);
+                hipError_t ce = hipSuccess;
+                if (ctx->clock_in_stages()) {
+                    ctx->enqueue_step(k, true, ce, true);
+                } else {
+                    ce = launch_clock_finalize(ctx->clock, 1, ctx->stream);
+                    if (ce == hipSuccess) ctx->enqueue_step(k, true, ce);
+                }
                 e = hipStreamEndCapture(ctx->stream, &g);
                 if (e == hipSuccess) e = ce;
                 if (e == hipSuccess) e = hipGraphInstantiate(&ctx->graph[k], g, nullptr, nullptr, 0);

@@ -294,6 +294,8 @@ struct mh_ctx {
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
                 set_virt(g);
+                g.fin = stage_fin;
+                g.clock = clock;
                 e = launch_stage2d_f32c(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
             } else if (nd == 3) {
@@ -308,6 +310,8 @@ struct mh_ctx {
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
                 set_virt(g);
+                g.fin = stage_fin;
+                g.clock = clock;
                 e = launch_stage2d_f64(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
             } else if (nd == 3) {
@@ -377,20 +381,29 @@ struct mh_ctx {
     // With a user_bc hook the reduction is a separate pass after the hook.
     // With virtual_ghosts() no stage fills ghosts: the caller fills those of
     // the last step's result (fill_final).
-    int enqueue_step(int k, bool reduce_last, hipError_t& e) {
+    // With fuse_clock (clock_in_stages()) the step's clock finalize runs
+    // inside its first two stages (StageGrid::fin) instead of a kernel of its
+    // own before them.
+    int enqueue_step(int k, bool reduce_last, hipError_t& e, bool fuse_clock = false) {
         const int ns = num_stages(sch.stepper);
         const bool fused = reduce_last && !user_fn;
         int in = k;
         for (int s = 0; s < ns; ++s) {
             last_in = in;
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
+            stage_fin = fuse_clock ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : 0;
             e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0, false);
+            stage_fin = 0;
             if (e != hipSuccess) return k;
             in = out;
         }
         if (reduce_last && !fused) e = cfl_all(in);
         return in;
     }
+    // 2-D steps of three or more stages (the second stage clears the CFL
+    // slots that only the last one folds into), no user_bc hook
+    bool clock_in_stages() const { return nd == 2 && num_stages(sch.stepper) >= 3 && !user_fn; }
+    int stage_fin = 0;  // StageGrid::fin of the next 2-D stage launch
 
     // ghosts of buffer k after steps run with virtual ghosts
     hipError_t fill_final(int k) { return virtual_ghosts() ? ghost(k, k, 0) : hipSuccess; }

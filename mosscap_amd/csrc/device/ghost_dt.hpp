@@ -62,6 +62,36 @@ __device__ inline void fold_amax(unsigned long long* slots, uint32_t block, uint
     atomicMax(slots + kAmaxStride * (block & (uint32_t)(kAmaxSlots - 1)), (unsigned long long)m);
 }
 
+// The next step's dt from the CFL slots and the clock (compute_dt_impl's
+// dt_max = dx / max(a), times max_cfl, with the output-time clamp and its
+// nextafter loop, Hydro.cpp:208-240). Called by every lane of one wave (lane
+// l reads slot l); every lane returns the same value. Reads only.
+__device__ inline double clock_next_dt(const Clock* c) {
+    static_assert(kAmaxSlots == kWave, "one slot per lane");
+    const unsigned long long* slot =
+        amax_slots(const_cast<Clock*>(c)) + kAmaxStride * (threadIdx.x & (unsigned)(kWave - 1));
+    const double amax = __builtin_bit_cast(double, wave_max_u64(*slot));
+    const double dt_min = c->dx / amax;  // == min over cells of dx / a
+    double dt = dt_min * c->max_cfl;
+    const double t = c->time;
+    const double next_write_time = smin(c->max_time, c->prev_output_time + c->delta);
+    if (t + dt >= next_write_time) {
+        dt = next_write_time - t;
+        while (t + dt < next_write_time) {
+            dt = nextafter(dt, 1.7976931348623157e308);
+        }
+    }
+    return dt;
+}
+
+// time += dt and, once an output time is reached, prev_output_time = time
+// (main.cpp:62-71). One thread.
+__device__ inline void clock_advance(Clock* c) {
+    c->time = c->time + c->dt;
+    c->step += 1;
+    if (c->time >= c->prev_output_time + c->delta) c->prev_output_time = c->time;
+}
+
 // Source index along an axis of extent `dims` for ghost coordinate `g`
 // (fill_one_bc_impl, Boundaries.hpp:27-57).
 __host__ __device__ inline int ghost_source(int g, int dims, int ng, BoundaryType b) {

@@ -79,6 +79,14 @@ struct StageGrid {
     // copy), so the stage needs no ghost fill of its input. xper / yper /
     // zper: periodic along that axis, else zerograd.
     int32_t virt = 0, xper = 0, yper = 0, zper = 0;
+    // The step's clock work folded into its stages (graph-replayed steps of
+    // three or more stages): 1 = this first stage takes the step's dt from
+    // the CFL slots and the clock itself (every wave, read-only) and block 0
+    // stores it in the clock; 2 = block 0 of this second stage advances the
+    // clock's time and clears the slots for the last stage's fold. 0 = the
+    // stage reads dt from the clock.
+    int32_t fin = 0;
+    Clock* clock = nullptr;
 };
 
 
@@ -168,7 +176,18 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         else return Form::get(p.qs, uniform + col);
     };
     const EosT<T> eos = make_eos<T>((double)p.gamma);
-    const T dt = (T)p.dt[0];
+    double dt_step;
+    if (g.fin == 1) {
+        dt_step = clock_next_dt(g.clock);
+        if (blockIdx.x == 0 && lane == 0) g.clock->dt = dt_step;
+    } else {
+        dt_step = p.dt[0];
+        if (g.fin == 2 && blockIdx.x == 0) {
+            if (lane == 0) clock_advance(g.clock);
+            amax_slots(g.clock)[kAmaxStride * lane] = 0ull;
+        }
+    }
+    const T dt = (T)dt_step;
     const T dtdx = dt / (T)p.dx;
     const T src = T(0) * dt;
 

@@ -1,4 +1,4 @@
-// Batched 1-D ensemble, compensated FP32 form (-ffp-contract=fast; the
+// Batched 1-D ensemble, compensated FP32 form (-ffp-contract=fast-honor-pragmas; the
 // error-free transforms pin contraction off themselves).
 #define MH_STAGE_WAVES_PER_EU 1
 // 5 waves per SIMD fit without spills with the next item's loads in flight

@@ -168,29 +168,12 @@ hipError_t probe_riemann_t(int solver, int axis, int nd, double gamma, int n, co
 // Resets the running max for the next step.
 // One wave: lane l reduces and clears slot l.
 __global__ void clock_finalize_kernel(Clock* c, int advance) {
-    static_assert(kAmaxSlots == 64, "one slot per lane");
-    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
-    unsigned long long* slot = amax_slots(c) + kAmaxStride * threadIdx.x;
-    const uint64_t bits = wave_max_u64(*slot);
-    *slot = 0ull;
+    if (blockIdx.x != 0 || threadIdx.x >= kWave) return;
+    const double dt = clock_next_dt(c);
+    amax_slots(c)[kAmaxStride * threadIdx.x] = 0ull;  // after every lane has read its slot (wave order)
     if (threadIdx.x != 0) return;
-    const double amax = __builtin_bit_cast(double, bits);
-    const double dt_min = c->dx / amax;  // == min over cells of dx / a
-    double dt = dt_min * c->max_cfl;
-    const double t = c->time;
-    const double next_write_time = smin(c->max_time, c->prev_output_time + c->delta);
-    if (t + dt >= next_write_time) {
-        dt = next_write_time - t;
-        while (t + dt < next_write_time) {
-            dt = nextafter(dt, 1.7976931348623157e308);
-        }
-    }
     c->dt = dt;
-    if (advance) {
-        c->time = t + dt;
-        c->step += 1;
-        if (c->time >= c->prev_output_time + c->delta) c->prev_output_time = c->time;
-    }
+    if (advance) clock_advance(c);
 }
 
 __global__ void clock_set_dt_kernel(Clock* c, double dt) {

@@ -1,5 +1,5 @@
 // 3-D stage kernels (z sweep + per-plane x-y march) of the compensated FP32
-// form, built with -ffp-contract=fast like the 2-D unit. Five variables per
+// form, built with -ffp-contract=fast-honor-pragmas like the 2-D unit. Five variables per
 // cell need more registers than the 2-D march: the plain instance is held to
 // at least three waves per SIMD (it takes 112 VGPRs, four waves), the one with
 // the gravity source to two.

@@ -1,8 +1,9 @@
 // Fused stage kernels of the compensated FP32 precision form. Built with
-// -ffp-contract=fast: multiply-adds may fuse, except inside the error-free
-// transforms of the stage combine (form_f32c.hpp pins those with a pragma).
-// The plain instance fits 128 VGPRs (4 waves per SIMD); the instance with
-// the gravity source and tracer stores needs more (3 waves).
+// -ffp-contract=fast-honor-pragmas: multiply-adds may fuse, except where
+// form_f32c.hpp pins contraction off (the stage combine's error-free
+// transforms, the CFL signal speed). The plain instance fits 128 VGPRs (4
+// waves per SIMD); the instance with the gravity source and tracer stores
+// needs more (3 waves).
 #ifndef MH_STAGE_WAVES_F32C
 #define MH_STAGE_WAVES_F32C 4
 #endif

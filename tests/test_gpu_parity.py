@@ -214,14 +214,20 @@ def test_kh_256_weno_hllc_ssprk3(orc):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_device_loop_equals_host_loop(orc, use_graph):
-    """mh_run (dt fused into the last stage, optional graph replay) == compute_dt + time_step."""
-    cfg = _kh_small(100, 64, "weno5z", "minmod", "hllc", "ssprk3", bc=("periodic", "periodic", "wall", "constant"))
-    cfg.bc_const = ((0,) * 5, (0,) * 5, (0,) * 5, (1.0, 0.3, -0.2, 9.0, 0))
+@pytest.mark.parametrize("form", [M.FP64, M.FP32C])
+def test_device_loop_equals_host_loop(orc, use_graph, form):
+    """mh_run (the CFL max fused into the last stage; graph-replayed steps of three or more stages
+    also take their dt and clock update inside their first two stages) == compute_dt + time_step,
+    through the output-time clamp. FP32C on a periodic grid (ghosts read from their sources)."""
+    if form == M.FP64:
+        cfg = _kh_small(100, 64, "weno5z", "minmod", "hllc", "ssprk3", bc=("periodic", "periodic", "wall", "constant"))
+        cfg.bc_const = ((0,) * 5, (0,) * 5, (0,) * 5, (1.0, 0.3, -0.2, 9.0, 0))
+    else:
+        cfg = _kh_small(100, 64, "weno5z", "minmod", "hllc", "ssprk3", bc=("periodic",) * 4)
     q0 = problems.kh_band(cfg, seed=5)
     for stepper in ("rk2", "ssprk3", "ssprk4"):
         cfg.time_stepper = stepper
-        with HydroContext(cfg) as a, HydroContext(cfg) as b:
+        with HydroContext(cfg, precision_form=form) as a, HydroContext(cfg, precision_form=form) as b:
             a.upload(q0)
             a.fill_bcs()
             b.upload(q0)
