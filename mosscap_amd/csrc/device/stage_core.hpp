@@ -51,6 +51,12 @@ constexpr int kStripOut = 62;  // output columns per wavefront
 #ifndef MH_STAGE_QS_RING
 #define MH_STAGE_QS_RING 0
 #endif
+// 3-D plain march: 1 = the z flux differences of the next row go straight
+// into LDS with the combine operands; 0 = they are loaded into registers at
+// the top of the row they are added in.
+#ifndef MH_STAGE_DZ_LDS
+#define MH_STAGE_DZ_LDS 0
+#endif
 
 // Stage update kinds: the lambdas of TimeStepping.cpp, src = S*dt with S = 0.
 enum StageKind : int32_t {
@@ -205,9 +211,25 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                     (MH_STAGE_QS_RING == 2 && R == Reconstruction::Weno5Z));
     constexpr int kQnSlot = kQsRing ? 0 : 1;
     __shared__ uint32_t opnd[kDmaOps && !kQsRing ? 2 : 1][kDmaOps ? NV : 1][2][kWave];  // [Q^(s), Q^n][var][word][lane]
+    // 3-D: the z flux differences of the combined row (the z sweep's output,
+    // one value per variable and interior cell; lanes outside read a clamped
+    // column that they do not use)
+    constexpr bool kDzLds = ND > 2 && kDmaOps && MH_STAGE_DZ_LDS != 0;
+    constexpr int kDzWords = sizeof(T) / 4;
+    __shared__ uint32_t dzo[kDzLds ? NV : 1][kDzLds ? kDzWords : 1][kWave];
+    const int c_dz = min(c, g.xc - 1);
     auto issue_operands = [&](int jr) {
         if constexpr (kDmaOps) {
             const int row = pk + jr * xc;
+            if constexpr (kDzLds) {
+                const auto bdz = make_rsrc(p.dz3);
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int w = 0; w < kDzWords; ++w)
+                        buf_load_lds(bdz, &dzo[v][w][0], (uint32_t)c_dz * (uint32_t)sizeof(T) + 4u * w,
+                                     (uint32_t)(v * plane + row) * (uint32_t)sizeof(T));
+            }
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if (!kQsRing)
@@ -388,6 +410,14 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             if (j + 1 < j1) fetch_row(j + 2 + ORD, qa);  // prefetch the next row
         }
 
+        // 3-D: this row's z flux differences into registers now, so the
+        // load has the sweeps to arrive (without kDzLds)
+        T dzr[ND > 2 && !kDzLds ? NV : 1];
+        if constexpr (ND > 2 && !kDzLds) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dzr[v] = p.dz3[v * plane + pk + j * xc + c_dz];
+        }
+
         // y sweep: cell j+1, face j+1
         T yl[NV], yr[NV], Gy[NV], dy[NV];
         recon_y(yl, yr);
@@ -460,7 +490,11 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 // drops the leading 0, which only turns -0 into +0)
                 T q_update = Form::kExactZeros ? T(0) + (Fx[v] - Fxp[v]) : Fx[v] - Fxp[v];
                 q_update += dy[v];
-                if constexpr (ND > 2) q_update += p.dz3[v * plane + pk + j * xc + c];
+                if constexpr (ND > 2) {
+                    if constexpr (!kDzLds) q_update += dzr[v];
+                    else if constexpr (kDzWords == 2) q_update += Form::from_words(dzo[v][0][lane], dzo[v][1][lane]);
+                    else q_update += __builtin_bit_cast(T, dzo[v][0][lane]);
+                }
                 q_update *= dtdx;
                 qo[v] = (Extras && p.gravity) ? Form::combine(g.kind, qs_c[v], qn_c[v], q_update, srcv[v])
                                               : Form::combine_nosrc(g.kind, qs_c[v], qn_c[v], q_update);

@@ -9,6 +9,10 @@
 #define MH_STAGE3D_WAVES_F64 2
 #endif
 #define MH_STAGE_LEAD_ROWS 2
+// the z flux differences of a row reach LDS with the combine operands
+#ifndef MH_STAGE_DZ_LDS
+#define MH_STAGE_DZ_LDS 1
+#endif
 #define MH_STAGE_WAVES_PER_EU MH_STAGE3D_WAVES_F64
 #define MH_STAGE3D_INSTANCES 1
 #include "stage_inst.hpp"
