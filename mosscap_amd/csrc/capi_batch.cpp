@@ -100,6 +100,9 @@ struct mh_batch {
         const BatchArrays b = arrays();
         if (mode == 0) {
             BatchGrid g{ng, nitems, nprob, kind, reduce, 0, 0};
+            g.fin = stage_fin;
+            g.max_cfl = fin_cfl;
+            g.max_time = fin_max_time;
             g.virt = virtual_ghosts() ? 1 : 0;
             g.per = bdry.xs == (int32_t)BoundaryType::Periodic;
             const int grid = std::min(waves > 0 ? waves : cus * 12, nitems);
@@ -137,17 +140,29 @@ struct mh_batch {
         return e;
     }
 
-    int enqueue_step(int mode, int k, double max_cfl, double max_time, hipError_t& e) {
-        e = launch_batch_finalize(nprob, max_cfl, max_time, 1, arrays(), d_dt, stream);
+    // With fuse_clock (clock_in_stages(mode)) the per-problem finalize runs
+    // inside the step's first two stages (BatchGrid::fin), not as a kernel.
+    int enqueue_step(int mode, int k, double max_cfl, double max_time, hipError_t& e, bool fuse_clock = false) {
+        e = fuse_clock ? hipSuccess : launch_batch_finalize(nprob, max_cfl, max_time, 1, arrays(), d_dt, stream);
         const int ns = num_stages(sch.stepper);
         int in = k;
+        fin_cfl = max_cfl;
+        fin_max_time = max_time;
         for (int s = 0; s < ns && e == hipSuccess; ++s) {
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
+            stage_fin = fuse_clock ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : 0;
             e = stage(mode, in, k, out, stage_kind_of(sch.stepper, s), s == ns - 1 ? 1 : 0);
+            stage_fin = 0;
             in = out;
         }
         return in;
     }
+    // persistent steps of three or more stages (the second clears the CFL
+    // maxima that only the last folds into)
+    bool clock_in_stages(int mode) const { return mode == 0 && num_stages(sch.stepper) >= 3; }
+    int stage_fin = 0;  // BatchGrid::fin of the next persistent stage launch
+    double fin_cfl = 0.0, fin_max_time = 0.0;
+    double graph_cfl = -1.0, graph_max_time = -1.0;  // the values the graphs were captured with
 
     void free_all() {
         drop_graphs();
@@ -335,6 +350,12 @@ int mh_batch_run(mh_batch* b, int64_t nsteps, double max_cfl, double max_time, i
         e = b->fp32c() ? launch_batch_cfl_f32c(b->nprob, b->p32(b->cur), b->gamma, arr, b->stream)
                        : launch_batch_cfl_f64(b->nprob, b->p64(b->cur), b->gamma, arr, b->stream);
     }
+    // the captured steps carry max_cfl and max_time as kernel arguments
+    if (use_graph && (max_cfl != b->graph_cfl || max_time != b->graph_max_time)) {
+        b->drop_graphs();
+        b->graph_cfl = max_cfl;
+        b->graph_max_time = max_time;
+    }
     const int advance = num_stages(b->sch.stepper) == 3 ? 1 : 2;  // buffer rotation per step
     for (int64_t s = 0; s < nsteps && e == hipSuccess; ++s) {
         const int k = b->cur;
@@ -345,7 +366,7 @@ int mh_batch_run(mh_batch* b, int64_t nsteps, double max_cfl, double max_time, i
                 e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) break;
                 hipError_t ce = hipSuccess;
-                b->enqueue_step(mode, k, max_cfl, max_time, ce);
+                b->enqueue_step(mode, k, max_cfl, max_time, ce, b->clock_in_stages(mode));
                 e = hipStreamEndCapture(b->stream, &g);
                 if (e == hipSuccess) e = ce;
                 if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);

@@ -18,7 +18,7 @@ struct BatchArrays {
     const int* base;            // element offset of each problem's block
     const int* xc;              // cells incl. ghosts
     const double* dx;
-    const double* dt;           // per-problem dt of the current step
+    double* dt;                 // per-problem dt of the current step
     unsigned long long* amax;   // per-problem CFL max (bit pattern)
     double* time;               // per-problem clock
 };
@@ -26,7 +26,28 @@ struct BatchArrays {
 struct BatchGrid {
     int32_t ng, nitems, nprob, kind, reduce_dt;
     int32_t virt, per;  // ghosts read from their sources (periodic `per`, else zerograd); no fill per stage
+    // The step's per-problem clock work folded into its stages (graph-replayed
+    // persistent steps of three or more stages): 1 = this first stage takes
+    // each problem's dt from its CFL max and clock itself and the wave with
+    // the problem's first segment stores it; 2 = that wave of this second
+    // stage advances the problem's time and clears its CFL max. 0 = none.
+    int32_t fin = 0;
+    double max_cfl = 0.0, max_time = 0.0;
 };
+
+// A problem's next dt from its CFL max and clock (compute_dt_impl's
+// dx / max(a) times max_cfl, clamped to end exactly at max_time with the
+// reference's nextafter loop). Reads only.
+__device__ inline double batch_next_dt(const BatchArrays& b, int pr, double max_cfl, double max_time) {
+    const double amax = __builtin_bit_cast(double, b.amax[pr]);
+    double dt = (b.dx[pr] / amax) * max_cfl;
+    const double t = b.time[pr];
+    if (t + dt >= max_time) {
+        dt = max_time - t;
+        while (t + dt < max_time) dt = nextafter(dt, 1.7976931348623157e308);
+    }
+    return dt;
+}
 
 // Each wave walks its items with the next item's loads issued before the
 // current item's arithmetic (the item carries the block offset and length,
@@ -53,7 +74,18 @@ __device__ inline void batch_stage(const BatchGrid& g, const typename Form::Ptrs
             seg_load<Form, R>(p, g.kind, wn.z, wn.w, g.ng, wn.w - 2 * g.ng, wn.y, virt, per, nxt);
         }
         const int pr = w.x, i0 = w.y, base = w.z, xc = w.w;
-        const T dt = (T)b.dt[pr];
+        double dt_step;
+        if (g.fin == 1) {
+            dt_step = batch_next_dt(b, pr, g.max_cfl, g.max_time);
+            if (i0 == g.ng && lane == 0) b.dt[pr] = dt_step;
+        } else {
+            dt_step = b.dt[pr];
+            if (g.fin == 2 && i0 == g.ng && lane == 0) {
+                b.time[pr] = b.time[pr] + dt_step;
+                b.amax[pr] = 0ull;
+            }
+        }
+        const T dt = (T)dt_step;
         const T dtdx = dt / (T)b.dx[pr];
         double amax = 0.0;
         seg_compute<Form, R, SL, RS>(p, g.kind, g.reduce_dt, base, xc, g.ng, xc - 2 * g.ng, i0, dtdx, dt, lds, amax,

@@ -10,13 +10,8 @@ __global__ void batch_finalize_kernel(int nprob, double max_cfl, double max_time
                                       double* dt_out) {
     const int pr = blockIdx.x * blockDim.x + threadIdx.x;
     if (pr >= nprob) return;
-    const double amax = __builtin_bit_cast(double, b.amax[pr]);
-    double dt = (b.dx[pr] / amax) * max_cfl;
+    const double dt = batch_next_dt(b, pr, max_cfl, max_time);
     const double t = b.time[pr];
-    if (t + dt >= max_time) {
-        dt = max_time - t;
-        while (t + dt < max_time) dt = nextafter(dt, 1.7976931348623157e308);
-    }
     dt_out[pr] = dt;
     b.amax[pr] = 0ull;
     if (advance) b.time[pr] = t + dt;

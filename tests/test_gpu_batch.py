@@ -118,3 +118,26 @@ def test_batch_autotune_restores_state(form):
         a.run(5)
         b.run(5)
         np.testing.assert_array_equal(a.download(), b.download())
+
+
+@pytest.mark.parametrize("stepper", ["rk2", "ssprk3", "ssprk4"])
+def test_batch_graph_follows_run_arguments(stepper):
+    """Graph-replayed persistent steps (for three or more stages the per-problem dt and clock
+    update run inside the first two stages) equal the launch-by-launch steps bit for bit, also
+    when max_cfl and max_time change between calls (the captured steps are rebuilt); a
+    problem that reaches max_time stops there (its later steps have dt = 0)."""
+    lengths, q0 = _small(nprob=30, seed=11, n_max=500)
+    calls = [(6, 0.4, 1e30), (5, 0.25, 1e30), (40, 0.4, 0.05), (3, 0.3, 0.05)]
+    out = []
+    for graph in (False, True):
+        with BatchContext(lengths, time_stepper=stepper) as b:
+            b.upload(q0)
+            b.fill_bcs()
+            for n, cfl, tmax in calls:
+                b.run(n, max_cfl=cfl, max_time=tmax, use_graph=graph)
+            times, dts = b.times()
+            out.append((b.download(), times, dts))
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])
+    assert np.any(out[0][1] == 0.05)  # some problems ended exactly at max_time
