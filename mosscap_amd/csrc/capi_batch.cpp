@@ -95,12 +95,14 @@ struct mh_batch {
     }
 
     // one stage for every problem: persistent (mode 0) or one launch per problem (mode 1)
-    hipError_t stage(int mode, int in, int qn_k, int out, int kind, int reduce) {
+    // fin, fin_cfl, fin_max_time: BatchGrid::fin and its arguments (persistent mode)
+    hipError_t stage(int mode, int in, int qn_k, int out, int kind, int reduce, int fin = 0, double fin_cfl = 0.0,
+                     double fin_max_time = 0.0) {
         hipError_t e = hipSuccess;
         const BatchArrays b = arrays();
         if (mode == 0) {
             BatchGrid g{ng, nitems, nprob, kind, reduce, 0, 0};
-            g.fin = stage_fin;
+            g.fin = fin;
             g.max_cfl = fin_cfl;
             g.max_time = fin_max_time;
             g.virt = virtual_ghosts() ? 1 : 0;
@@ -146,13 +148,10 @@ struct mh_batch {
         e = fuse_clock ? hipSuccess : launch_batch_finalize(nprob, max_cfl, max_time, 1, arrays(), d_dt, stream);
         const int ns = num_stages(sch.stepper);
         int in = k;
-        fin_cfl = max_cfl;
-        fin_max_time = max_time;
         for (int s = 0; s < ns && e == hipSuccess; ++s) {
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
-            stage_fin = fuse_clock ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : 0;
-            e = stage(mode, in, k, out, stage_kind_of(sch.stepper, s), s == ns - 1 ? 1 : 0);
-            stage_fin = 0;
+            const int fin = fuse_clock ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : 0;
+            e = stage(mode, in, k, out, stage_kind_of(sch.stepper, s), s == ns - 1 ? 1 : 0, fin, max_cfl, max_time);
             in = out;
         }
         return in;
@@ -160,8 +159,6 @@ struct mh_batch {
     // persistent steps of three or more stages (the second clears the CFL
     // maxima that only the last folds into)
     bool clock_in_stages(int mode) const { return mode == 0 && num_stages(sch.stepper) >= 3; }
-    int stage_fin = 0;  // BatchGrid::fin of the next persistent stage launch
-    double fin_cfl = 0.0, fin_max_time = 0.0;
     double graph_cfl = -1.0, graph_max_time = -1.0;  // the values the graphs were captured with
 
     void free_all() {

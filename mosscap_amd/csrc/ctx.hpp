@@ -239,11 +239,12 @@ struct mh_ctx {
     // yet, and Q^n keeps the values it had.
     // With `fill` false and virtual_ghosts(), the output's ghosts are left
     // stale: the next stage reads ghost rows and columns from their sources.
-    hipError_t stage(int in, int qn_k, int out, int kind, int reduce, bool fill = true) {
+    // fin: StageGrid::fin of a 2-D stage (the step's clock work inside it)
+    hipError_t stage(int in, int qn_k, int out, int kind, int reduce, bool fill = true, int fin = 0) {
         const int src = !tab_on ? in : (in == qn_k ? 3 - in - out : in);
         hipError_t e = timed(0, [&] {
             hipError_t r = tab_on ? tab_update(in, src) : hipSuccess;
-            if (r == hipSuccess) r = stage_kernel(src, qn_k, out, kind, reduce);
+            if (r == hipSuccess) r = stage_kernel(src, qn_k, out, kind, reduce, fin);
             return r;
         });
         if (e != hipSuccess || (!fill && virtual_ghosts())) return e;
@@ -284,7 +285,7 @@ struct mh_ctx {
         return g;
     }
 
-    hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce) {
+    hipError_t stage_kernel(int in, int qn_k, int out, int kind, int reduce, int fin = 0) {
         hipError_t e;
         const TracerGrid tg{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, ntr, kind};
         if (fp32c()) {
@@ -294,7 +295,7 @@ struct mh_ctx {
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
                 set_virt(g);
-                g.fin = stage_fin;
+                g.fin = fin;
                 g.clock = clock;
                 e = launch_stage2d_f32c(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f32c(sch, tg, p, wtr32, stream);
@@ -310,7 +311,7 @@ struct mh_ctx {
             if (nd == 2) {
                 StageGrid g{sz.xc, sz.yc, sz.ng, sz.xc - 2 * sz.ng, sz.yc - 2 * sz.ng, strips, nspans, kind, reduce};
                 set_virt(g);
-                g.fin = stage_fin;
+                g.fin = fin;
                 g.clock = clock;
                 e = launch_stage2d_f64(sch, g, p, stream);
                 if (e == hipSuccess && ntr > 0) e = launch_tracer_f64(sch, tg, p, wtr64, stream);
@@ -391,9 +392,8 @@ struct mh_ctx {
         for (int s = 0; s < ns; ++s) {
             last_in = in;
             const int out = (in == k) ? (k + 1) % 3 : 3 - in - k;
-            stage_fin = fuse_clock ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : 0;
-            e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0, false);
-            stage_fin = 0;
+            const int fin = fuse_clock ? (s == 0 ? 1 : (s == 1 ? 2 : 0)) : 0;
+            e = stage(in, k, out, stage_kind(sch.stepper, s), (fused && s == ns - 1) ? 1 : 0, false, fin);
             if (e != hipSuccess) return k;
             in = out;
         }
@@ -403,7 +403,6 @@ struct mh_ctx {
     // 2-D steps of three or more stages (the second stage clears the CFL
     // slots that only the last one folds into), no user_bc hook
     bool clock_in_stages() const { return nd == 2 && num_stages(sch.stepper) >= 3 && !user_fn; }
-    int stage_fin = 0;  // StageGrid::fin of the next 2-D stage launch
 
     // ghosts of buffer k after steps run with virtual ghosts
     hipError_t fill_final(int k) { return virtual_ghosts() ? ghost(k, k, 0) : hipSuccess; }
