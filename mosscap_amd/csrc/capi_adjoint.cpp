@@ -142,6 +142,9 @@ int mh_adjoint_backward(mh_ctx* ctx, const double* lambda_final, double* lambda_
     std::memcpy(ctx->adj_host, lambda_final, nv * sizeof(double));
     e = hipMemcpyAsync(cur.p, ctx->adj_host, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, cur.p, 0.0, cur.p, ctx->stream);  // clear ghost entries
+    // ... of the other cotangent buffer too: the line sweeps of a periodic
+    // axis never write its ghost entries, which must read as zero
+    if (e == hipSuccess) e = launch_adjoint_axpby(g, 1.0, nxt.p, 0.0, nxt.p, ctx->stream);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);

@@ -44,6 +44,16 @@ __device__ inline float lane_from_right(float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, x)));
 }
 
+// Value of x in lane `src` (clamped to the wave).
+__device__ inline double lane_value(double x, int src) {
+    const int a = min(max(src, 0), kWave - 1) << 2;
+    const int2 v = __builtin_bit_cast(int2, x);
+    int2 r;
+    r.x = __builtin_amdgcn_ds_bpermute(a, v.x);
+    r.y = __builtin_amdgcn_ds_bpermute(a, v.y);
+    return __builtin_bit_cast(double, r);
+}
+
 // Wavefront max of a non-negative 64-bit pattern (non-negative doubles order
 // like their bit patterns).
 __device__ inline uint64_t wave_max_u64(uint64_t x) {

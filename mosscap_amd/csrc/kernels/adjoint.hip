@@ -19,7 +19,10 @@
 // applies the cons->prim VJP and the stage combination, and writes every cell
 // of the output (ghost cells get a_l * their cotangent and are folded into
 // their sources by the transposed ghost fill afterwards: y ghosts, then x
-// ghosts, the reverse of the fill).
+// ghosts, the reverse of the fill). An axis periodic on both faces runs its
+// lines as circles instead: a wave's halo positions past either end read the
+// periodic images, so the faces across the seam are differentiated by the
+// waves that own the cells on either side and no cotangent reaches a ghost.
 #include "../device/adjoint.hpp"
 #include "../device/ghost_dt.hpp"
 #include "../device/wave.hpp"
@@ -41,6 +44,10 @@ struct LineArgs {
     double a_qn, a_qs, a_l;
     int32_t acc_mode;  // AdjStage::acc_mode
     int32_t nseg;
+    // the axis is periodic on both faces: its lines are circular (positions
+    // run over the interior, halo positions wrap), so no contribution lands
+    // in a ghost cell and that axis needs no transposed ghost fill
+    int32_t circ_x, circ_y;
 };
 
 template <RiemannSolver RS, int Axis>
@@ -56,16 +63,18 @@ __device__ inline void face_vjp(double gamma, const double (&wl)[4], const doubl
 }
 
 // One wavefront per (line, segment). Axis 0: lines are the interior rows,
-// positions run over all columns. Axis 1: lines are all columns (ghost
-// columns only carry the x sweep's cotangents to the epilogue), positions
-// run over all rows.
+// positions run over all columns (over the interior columns, wrapping, when x
+// is circular). Axis 1: lines are all columns (ghost columns only carry the x
+// sweep's cotangents to the epilogue; the interior columns alone when x is
+// circular), positions run over all rows (the interior rows, wrapping, when
+// y is circular).
 template <Reconstruction R, SlopeLimiter SL, RiemannSolver RS, int Axis>
 __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
     constexpr int ORD = stencil_order(R);
     constexpr int HALO = 2 * ORD + 1;
     constexpr int OUT = kWave - 2 * HALO;
+    // 8 KB of LDS per wave, so that 4 waves fit each SIMD
     __shared__ double sW[4][kWave], sWR[4][kWave], sMu[4][kWave], sLR[4][kWave];
-    __shared__ double sC[2 * ORD + 1][4][kWave];
 
     const AdjGrid& g = a.g;
     const int t = threadIdx.x;
@@ -80,9 +89,12 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
     const int line = Axis == 0 ? (int)(lb / (uint32_t)a.nseg) : (int)(lb % nlines);
     const int len = Axis == 0 ? g.xc : g.yc;
     const int nax = Axis == 0 ? g.nx : g.ny;
-    const int p = seg * OUT - HALO + t;  // position along the line
-    const int pc = min(max(p, 0), len - 1);
-    const int li = Axis == 0 ? g.ng + line : line;  // row (axis 0) or column (axis 1)
+    const bool circ = (Axis == 0 ? a.circ_x : a.circ_y) != 0;
+    // position along the line (with ghosts); circular lines number the
+    // interior from 0 and read a position outside it at its periodic image
+    const int p = circ ? g.ng + seg * OUT - HALO + t : seg * OUT - HALO + t;
+    const int pc = circ ? g.ng + ((p - g.ng) % nax + nax) % nax : min(max(p, 0), len - 1);
+    const int li = Axis == 0 ? g.ng + line : (a.circ_x ? g.ng + line : line);  // row (axis 0) or column (axis 1)
     const bool line_in = Axis == 0 ? true : (li >= g.ng && li < g.ng + g.nx);
     const int cell = Axis == 0 ? li * g.xc + pc : pc * g.xc + li;
     const int i = Axis == 0 ? pc : li, j = Axis == 0 ? li : pc;
@@ -110,7 +122,7 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
     wave_lds_sync();
 
     // reconstruction of the cell at p (cells ng-1 .. ng+n along the axis)
-    const bool rec_ok = line_in && t >= ORD && t <= kWave - 1 - ORD && p >= g.ng - 1 && p <= g.ng + nax;
+    const bool rec_ok = line_in && t >= ORD && t <= kWave - 1 - ORD && (circ || (p >= g.ng - 1 && p <= g.ng + nax));
     double wl[4] = {0.0, 0.0, 0.0, 0.0}, wr[4] = {0.0, 0.0, 0.0, 0.0};
     if (rec_ok) {
 #pragma unroll
@@ -125,7 +137,7 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
 
     // face at the left edge of p (faces ng .. ng+n): RR of p-1 against RL of p
     double lL[4] = {0.0, 0.0, 0.0, 0.0};
-    if (line_in && t >= ORD + 1 && t <= kWave - 1 - ORD && p >= g.ng && p <= g.ng + nax) {
+    if (line_in && t >= ORD + 1 && t <= kWave - 1 - ORD && (circ || (p >= g.ng && p <= g.ng + nax))) {
         double wrl[4], lf[4], lrl[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -138,8 +150,11 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
     }
     wave_lds_sync();
 
-    // reconstruction VJP of the cell at p into its stencil
+    // reconstruction VJP of the cell at p into its stencil, and per cell the
+    // sum of the contributions aimed at it: stencil entry o of the cell in
+    // lane t - (o - ORD), read across lanes, summed in order of o
     const bool vjp_ok = rec_ok && t >= ORD + 1 && t <= kWave - 2 - ORD;
+    double lw[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         double ls[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -153,20 +168,14 @@ __global__ __launch_bounds__(64) void adj_line_kernel(LineArgs a) {
                 reconstruct_vjp_dual<R, SL>(st, lL[v], sLR[v][t], ls);
             }
         }
-#pragma unroll
-        for (int o = 0; o < 2 * ORD + 1; ++o) sC[o][v][t] = ls[2 - ORD + o];
-    }
-    wave_lds_sync();
-
-    if (t < HALO || t >= HALO + OUT || p >= len) return;
-    double lw[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
         double sum = 0.0;
 #pragma unroll
-        for (int o = 0; o < 2 * ORD + 1; ++o) sum += sC[o][v][t - (o - ORD)];
+        for (int o = 0; o < 2 * ORD + 1; ++o)
+            sum += o == ORD ? ls[2] : lane_value(ls[2 - ORD + o], t - (o - ORD));
         lw[v] = sum;
     }
+
+    if (t < HALO || t >= HALO + OUT || p >= (circ ? g.ng + nax : len)) return;
     if constexpr (Axis == 0) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) a.lamx[v * nc + cell] = lw[v];
@@ -275,13 +284,21 @@ __global__ void adj_axpby_kernel(AdjGrid g, double a, const double* __restrict__
 template <RiemannSolver RS, Reconstruction R, SlopeLimiter SL>
 hipError_t stage_impl(const AdjGrid& g, const AdjStage& st, hipStream_t stream) {
     constexpr int OUT = kWave - 2 * (2 * stencil_order(R) + 1);
-    LineArgs a{g, st.q, st.mu, st.lamx, st.nxt, st.acc, st.a_qn, st.a_qs, st.a_l, st.acc_mode, 0};
-    a.nseg = (g.xc + OUT - 1) / OUT;
+    LineArgs a{g, st.q, st.mu, st.lamx, st.nxt, st.acc, st.a_qn, st.a_qs, st.a_l, st.acc_mode, 0, 0, 0};
+    const auto per = (int32_t)BoundaryType::Periodic;
+    a.circ_x = g.bc[0] == per && g.bc[1] == per;
+    a.circ_y = g.bc[2] == per && g.bc[3] == per;
+    a.nseg = ((a.circ_x ? g.nx : g.xc) + OUT - 1) / OUT;
     hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 0>), dim3(a.nseg * g.ny), dim3(kWave), 0, stream, a);
-    a.nseg = (g.yc + OUT - 1) / OUT;
-    hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 1>), dim3(a.nseg * g.xc), dim3(kWave), 0, stream, a);
-    hipLaunchKernelGGL(adj_fold_kernel<1>, dim3((4 * g.xc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
-    hipLaunchKernelGGL(adj_fold_kernel<0>, dim3((4 * g.yc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
+    a.nseg = ((a.circ_y ? g.ny : g.yc) + OUT - 1) / OUT;
+    hipLaunchKernelGGL((adj_line_kernel<R, SL, RS, 1>), dim3(a.nseg * (a.circ_x ? g.nx : g.xc)), dim3(kWave), 0, stream,
+                       a);
+    // ghost cotangents of the non-circular axes into their sources (the
+    // ghost entries of a circular axis are never written and stay zero)
+    if (!a.circ_y)
+        hipLaunchKernelGGL(adj_fold_kernel<1>, dim3((4 * g.xc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
+    if (!a.circ_x)
+        hipLaunchKernelGGL(adj_fold_kernel<0>, dim3((4 * g.yc + kB - 1) / kB), dim3(kB), 0, stream, g, st.nxt);
     return hipGetLastError();
 }
 
