@@ -7,11 +7,14 @@
 //   and sqrt.
 // * cons->prim recovers the pressure p = (gamma-1)(E - |m|^2/(2 rho)) in FP64
 //   from the full hi+lo values: at high Mach number E and the kinetic energy
-//   nearly cancel, which FP32 cannot resolve.
+//   nearly cancel, which FP32 cannot resolve. A cell whose kinetic energy is
+//   at most a quarter of E (mild cancellation) takes it in FP32 in the march
+//   without floors.
 // * The stage combine adds each stage's increment over Q^n, formed in FP32,
 //   to the hi+lo Q^n with TwoSum, so the state accumulates the small
 //   per-stage increments without FP32 rounding drift.
-// * The CFL signal speed is evaluated in FP64 from hi+lo.
+// * The CFL signal speed is evaluated in FP64 from hi+lo, or in FP32 for a
+//   cell of moderate kinetic energy.
 // * Optional positivity floors on rho and p of the reconstruction input and
 //   of each stage's new state; each interior cell and stage that needed one
 //   is counted once.
@@ -19,6 +22,13 @@
 
 #include "functors.hpp"
 #include "wave.hpp"
+
+// 1: the plain instance's cons->prim and the CFL signal speed take cells of
+// moderate kinetic energy in FP32 (see to_prim_nd, signal_speed_nd); 0: every
+// cell through the FP64 recovery.
+#ifndef MH_F32C_FAST_PRIM
+#define MH_F32C_FAST_PRIM 1
+#endif
 
 namespace mh {
 
@@ -133,6 +143,32 @@ struct FormF32c {
     // density went non-positive still yields finite velocities.
     template <int ND, bool Floors = true, int NV>
     __device__ static inline bool to_prim_nd(const Ptrs& p, const float2 (&q)[NV], float (&w)[NV]) {
+#if MH_F32C_FAST_PRIM
+        if constexpr (!Floors) {
+            // Without floors, a cell whose kinetic energy is at most a
+            // quarter of E takes its primitives in FP32 from the hi words
+            // (and E's lo word): E - |m|^2/(2 rho) then loses no more than
+            // ~2 ulp, the FP32 rounding its primitives carry anyway. Other
+            // cells (high Mach, where the two nearly cancel) take the FP64
+            // recovery below.
+            const float rho = q[0].x;
+            const float r = __builtin_amdgcn_rcpf(rho);
+            float v[ND], kin2 = 0.0f;
+#pragma unroll
+            for (int a = 0; a < ND; ++a) {
+                v[a] = q[1 + a].x * r;
+                kin2 = __builtin_fmaf(q[1 + a].x, v[a], kin2);
+            }
+            const float E = q[1 + ND].x;
+            if (kin2 <= 0.5f * E) {
+                w[0] = rho;
+#pragma unroll
+                for (int a = 0; a < ND; ++a) w[1 + a] = v[a];
+                w[1 + ND] = (float)p.gm1 * ((E - 0.5f * kin2) + q[1 + ND].y);
+                return false;
+            }
+        }
+#endif
         double rho_d = df::to_double(q[0]);
         bool hit = false;
         const bool floors = Floors && p.rho_floor > 0.0f;
@@ -285,6 +321,23 @@ struct FormF32c {
         // Not contracted, so the stage kernels (built with contraction) and
         // the ghost and CFL kernels (built without) give a cell the same speed.
 #pragma clang fp contract(off)
+#if MH_F32C_FAST_PRIM
+        {
+            // A cell of moderate kinetic energy (at most a quarter of E, as in
+            // to_prim_nd) with a normal density and positive pressure: the
+            // speed in FP32 from the hi words (relative error ~1e-7, which
+            // moves dt by as much; the FP32 fluxes carry that anyway).
+            const float rho = q[0].x;
+            const float r = __builtin_amdgcn_rcpf(rho);
+            float m2 = 0.0f;
+#pragma unroll
+            for (int a = 0; a < ND; ++a) m2 = m2 + q[1 + a].x * q[1 + a].x;
+            const float kin2 = m2 * r, E = q[1 + ND].x;
+            const float pres = (float)(gamma - 1.0) * ((E - 0.5f * kin2) + q[1 + ND].y);
+            if (rho > 1e-30f && rho < 1e30f && kin2 <= 0.5f * E && pres > 0.0f)
+                return (double)(__builtin_amdgcn_sqrtf((float)gamma * pres * r) + __builtin_amdgcn_sqrtf(m2) * r);
+        }
+#endif
         // FP64 throughout, with 1/rho from the FP32 reciprocal and one Newton
         // step (relative error ~1e-14) in place of three IEEE divisions, and
         // the square roots from sqrt_fast: |v| = sqrt(|m|^2) / rho.
