@@ -33,6 +33,7 @@
 #include "functors.hpp"
 #include "ghost_dt.hpp"
 #include "wave.hpp"
+#include <utility>
 
 namespace mh {
 
@@ -54,6 +55,11 @@ constexpr int kStripOut = 62;  // output columns per wavefront
 // 3-D plain march: 1 = the z flux differences of the next row go straight
 // into LDS with the combine operands; 0 = they are loaded into registers at
 // the top of the row they are added in.
+// 1: the march runs its rows in groups of the Q^(s) ring's length (the ring
+// rotates by renaming); 0: one row at a time with the ring's registers moved.
+#ifndef MH_STAGE_RING_UNROLL
+#define MH_STAGE_RING_UNROLL 0
+#endif
 #ifndef MH_STAGE_DZ_LDS
 #define MH_STAGE_DZ_LDS 0
 #endif
@@ -373,7 +379,11 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
 
     double amax = 0.0;
 
-    for (int j = j0; j < j1; ++j) {
+    // one row of the march; rot: the Q^(s) ring's rotation (slot (k + rot) %
+    // (ORD + 1) holds row j + k), or -1 to keep slot k = row j + k by moving
+    // the ring's registers at the end of the row
+    auto march_row = [&](const int j, auto rot_c) {
+        constexpr int rot = decltype(rot_c)::value;
         // Stage-combine operands of this row: from LDS in the plain instance
         // (loaded during the previous row); otherwise loaded here or just
         // before the combine (Form::kEarlyOperands).
@@ -467,7 +477,7 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
             wave_lds_sync();
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                if constexpr (kQsRing) qs_c[v] = qsr[0][v];
+                if constexpr (kQsRing) qs_c[v] = qsr[rot < 0 ? 0 : rot][v];
                 else qs_c[v] = Form::from_words(opnd[0][v][0][lane], opnd[0][v][1][lane]);
                 qn_c[v] = Form::needs_qn(g.kind)
                               ? Form::from_words(opnd[kQnSlot][v][0][lane], opnd[kQnSlot][v][1][lane])
@@ -514,13 +524,16 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
         }
         if constexpr (kQsRing) {  // drop row j, take row j+1+ORD (committed at this row's top)
             if (j + 1 < j1) {
+                if constexpr (rot < 0) {
 #pragma unroll
-                for (int k = 0; k < ORD; ++k)
+                    for (int k = 0; k < ORD; ++k)
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) qsr[k][v] = qsr[k + 1][v];
+                        for (int v = 0; v < NV; ++v) qsr[k][v] = qsr[k + 1][v];
+                }
                 const int sl = lead_slot(j + 1 + ORD);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) qsr[ORD][v] = Form::from_words(lead[sl][v][0][lane], lead[sl][v][1][lane]);
+                for (int v = 0; v < NV; ++v)
+                    qsr[rot < 0 ? ORD : rot][v] = Form::from_words(lead[sl][v][0][lane], lead[sl][v][1][lane]);
             }
         }
         if constexpr (kDmaOps) {
@@ -530,6 +543,19 @@ __device__ inline void stage2d_march(const StageGrid& g, const typename Form::Pt
                 issue_operands(j + 1);
             }
         }
+    };
+    if constexpr (kQsRing && Inst == kStagePlain && MH_STAGE_RING_UNROLL) {
+        // rows in groups of the ring's length: each copy of the row addresses
+        // fixed slots, so the ring rotates without moving registers
+        constexpr int kRing = ORD + 1;
+        int j = j0;
+        for (; j + kRing <= j1; j += kRing)
+            [&]<int... I>(std::integer_sequence<int, I...>) {
+                (march_row(j + I, std::integral_constant<int, I>{}), ...);
+            }(std::make_integer_sequence<int, kRing>{});
+        for (; j < j1; ++j) march_row(j, std::integral_constant<int, -1>{});
+    } else {
+        for (int j = j0; j < j1; ++j) march_row(j, std::integral_constant<int, -1>{});
     }
 
     if (g.reduce_dt) {
