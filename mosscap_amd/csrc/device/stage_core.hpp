@@ -52,14 +52,15 @@ constexpr int kStripOut = 62;  // output columns per wavefront
 #ifndef MH_STAGE_QS_RING
 #define MH_STAGE_QS_RING 0
 #endif
-// 3-D plain march: 1 = the z flux differences of the next row go straight
-// into LDS with the combine operands; 0 = they are loaded into registers at
-// the top of the row they are added in.
-// 1: the march runs its rows in groups of the Q^(s) ring's length (the ring
-// rotates by renaming); 0: one row at a time with the ring's registers moved.
+// Plain instance with the Q^(s) ring: 1 = the march runs its rows in groups
+// of the ring's length (the ring rotates by renaming); 0 = one row at a time
+// with the ring's registers moved.
 #ifndef MH_STAGE_RING_UNROLL
 #define MH_STAGE_RING_UNROLL 0
 #endif
+// 3-D plain march: 1 = the z flux differences of the next row go straight
+// into LDS with the combine operands; 0 = they are loaded into registers at
+// the top of the row they are added in.
 #ifndef MH_STAGE_DZ_LDS
 #define MH_STAGE_DZ_LDS 0
 #endif
