@@ -7,9 +7,9 @@
 //   and sqrt.
 // * cons->prim recovers the pressure p = (gamma-1)(E - |m|^2/(2 rho)) in FP64
 //   from the full hi+lo values: at high Mach number E and the kinetic energy
-//   nearly cancel, which FP32 cannot resolve. A cell whose kinetic energy is
-//   at most a quarter of E (mild cancellation) takes it in FP32 in the march
-//   without floors.
+//   nearly cancel, which FP32 cannot resolve. Without floors, a cell whose
+//   kinetic energy is at most a quarter of E (mild cancellation) takes it in
+//   FP32.
 // * The stage combine adds each stage's increment over Q^n, formed in FP32,
 //   to the hi+lo Q^n with TwoSum, so the state accumulates the small
 //   per-stage increments without FP32 rounding drift.
@@ -23,9 +23,9 @@
 #include "functors.hpp"
 #include "wave.hpp"
 
-// 1: the plain instance's cons->prim and the CFL signal speed take cells of
-// moderate kinetic energy in FP32 (see to_prim_nd, signal_speed_nd); 0: every
-// cell through the FP64 recovery.
+// 1: cons->prim and the CFL signal speed take cells of moderate kinetic
+// energy in FP32 (see to_prim_nd, signal_speed_nd); 0: every cell through the
+// FP64 recovery.
 #ifndef MH_F32C_FAST_PRIM
 #define MH_F32C_FAST_PRIM 1
 #endif
@@ -144,13 +144,15 @@ struct FormF32c {
     template <int ND, bool Floors = true, int NV>
     __device__ static inline bool to_prim_nd(const Ptrs& p, const float2 (&q)[NV], float (&w)[NV]) {
 #if MH_F32C_FAST_PRIM
-        if constexpr (!Floors) {
-            // Without floors, a cell whose kinetic energy is at most a
-            // quarter of E takes its primitives in FP32 from the hi words
-            // (and E's lo word): E - |m|^2/(2 rho) then loses no more than
-            // ~2 ulp, the FP32 rounding its primitives carry anyway. Other
-            // cells (high Mach, where the two nearly cancel) take the FP64
-            // recovery below.
+        {
+            // Without floors, a cell whose kinetic energy is at most a quarter
+            // of E takes its primitives in FP32 from the hi words (and E's lo
+            // word): E - |m|^2/(2 rho) then loses no more than ~2 ulp, the
+            // FP32 rounding its primitives carry anyway. Other cells (high
+            // Mach, where the two nearly cancel) take the FP64 recovery
+            // below, and so does every cell with the floors on (taking the
+            // FP32 path where a cell is clear of them slowed the Sedov run,
+            // whose waves then ran both paths).
             const float rho = q[0].x;
             const float r = __builtin_amdgcn_rcpf(rho);
             float v[ND], kin2 = 0.0f;
@@ -160,11 +162,12 @@ struct FormF32c {
                 kin2 = __builtin_fmaf(q[1 + a].x, v[a], kin2);
             }
             const float E = q[1 + ND].x;
-            if (kin2 <= 0.5f * E) {
+            const float pres = (float)p.gm1 * ((E - 0.5f * kin2) + q[1 + ND].y);
+            if (kin2 <= 0.5f * E && !(Floors && p.rho_floor > 0.0f)) {
                 w[0] = rho;
 #pragma unroll
                 for (int a = 0; a < ND; ++a) w[1 + a] = v[a];
-                w[1 + ND] = (float)p.gm1 * ((E - 0.5f * kin2) + q[1 + ND].y);
+                w[1 + ND] = pres;
                 return false;
             }
         }
