@@ -7,9 +7,9 @@
 //   and sqrt.
 // * cons->prim recovers the pressure p = (gamma-1)(E - |m|^2/(2 rho)) in FP64
 //   from the full hi+lo values: at high Mach number E and the kinetic energy
-//   nearly cancel, which FP32 cannot resolve. Without floors, a cell whose
-//   kinetic energy is at most a quarter of E (mild cancellation) takes it in
-//   FP32.
+//   nearly cancel, which FP32 cannot resolve. A cell whose kinetic energy is
+//   at most half of E (mild cancellation), and at least twice above the
+//   floors when they are on, takes it in FP32.
 // * The stage combine adds each stage's increment over Q^n, formed in FP32,
 //   to the hi+lo Q^n with TwoSum, so the state accumulates the small
 //   per-stage increments without FP32 rounding drift.
@@ -28,6 +28,15 @@
 // FP64 recovery.
 #ifndef MH_F32C_FAST_PRIM
 #define MH_F32C_FAST_PRIM 1
+#endif
+// |m|^2/rho (twice the kinetic energy) at most MH_F32C_FAST_KIN * E for the
+// FP32 path; with floors, MH_F32C_FAST_FLOORS = 1 lets a cell at least twice
+// above both floors take it too (0: every cell of a floored run takes FP64).
+#ifndef MH_F32C_FAST_KIN
+#define MH_F32C_FAST_KIN 1.0f
+#endif
+#ifndef MH_F32C_FAST_FLOORS
+#define MH_F32C_FAST_FLOORS 1
 #endif
 
 namespace mh {
@@ -145,14 +154,14 @@ struct FormF32c {
     __device__ static inline bool to_prim_nd(const Ptrs& p, const float2 (&q)[NV], float (&w)[NV]) {
 #if MH_F32C_FAST_PRIM
         {
-            // Without floors, a cell whose kinetic energy is at most a quarter
-            // of E takes its primitives in FP32 from the hi words (and E's lo
-            // word): E - |m|^2/(2 rho) then loses no more than ~2 ulp, the
-            // FP32 rounding its primitives carry anyway. Other cells (high
-            // Mach, where the two nearly cancel) take the FP64 recovery
-            // below, and so does every cell with the floors on (taking the
-            // FP32 path where a cell is clear of them slowed the Sedov run,
-            // whose waves then ran both paths).
+            // A cell whose kinetic energy is at most half of E takes its
+            // primitives in FP32 from the hi words (and E's lo word):
+            // E - |m|^2/(2 rho) then loses no more than ~4 ulp, a few times
+            // the FP32 rounding its primitives carry anyway. With the floors
+            // on, only a cell at least twice above both (where the FP64 path
+            // would not floor it either). Other cells (high Mach, where the
+            // two nearly cancel, or near the floors) take the FP64 recovery
+            // below.
             const float rho = q[0].x;
             const float r = __builtin_amdgcn_rcpf(rho);
             float v[ND], kin2 = 0.0f;
@@ -163,7 +172,9 @@ struct FormF32c {
             }
             const float E = q[1 + ND].x;
             const float pres = (float)p.gm1 * ((E - 0.5f * kin2) + q[1 + ND].y);
-            if (kin2 <= 0.5f * E && !(Floors && p.rho_floor > 0.0f)) {
+            const bool floors = Floors && p.rho_floor > 0.0f;
+            const bool clear = MH_F32C_FAST_FLOORS ? (rho > 2.0f * p.rho_floor && pres > 2.0f * p.p_floor) : false;
+            if (kin2 <= MH_F32C_FAST_KIN * E && (!floors || clear)) {
                 w[0] = rho;
 #pragma unroll
                 for (int a = 0; a < ND; ++a) w[1 + a] = v[a];
@@ -326,7 +337,7 @@ struct FormF32c {
 #pragma clang fp contract(off)
 #if MH_F32C_FAST_PRIM
         {
-            // A cell of moderate kinetic energy (at most a quarter of E, as in
+            // A cell of moderate kinetic energy (at most half of E, as in
             // to_prim_nd) with a normal density and positive pressure: the
             // speed in FP32 from the hi words (relative error ~1e-7, which
             // moves dt by as much; the FP32 fluxes carry that anyway).
@@ -337,7 +348,7 @@ struct FormF32c {
             for (int a = 0; a < ND; ++a) m2 = m2 + q[1 + a].x * q[1 + a].x;
             const float kin2 = m2 * r, E = q[1 + ND].x;
             const float pres = (float)(gamma - 1.0) * ((E - 0.5f * kin2) + q[1 + ND].y);
-            if (rho > 1e-30f && rho < 1e30f && kin2 <= 0.5f * E && pres > 0.0f)
+            if (rho > 1e-30f && rho < 1e30f && kin2 <= MH_F32C_FAST_KIN * E && pres > 0.0f)
                 return (double)(__builtin_amdgcn_sqrtf((float)gamma * pres * r) + __builtin_amdgcn_sqrtf(m2) * r);
         }
 #endif
