@@ -30,7 +30,8 @@ extern "C" {
 
 #define MH_FORM_FP64 0  /* drop-in: FP64 state and arithmetic, bitwise the reference CPU build */
 #define MH_FORM_FP32C 1 /* precision form: FP32 reconstruction/Riemann, compensated hi+lo FP32 state,
-                           FP64 pressure recovery and CFL reduction */
+                           pressure recovery and CFL speed in FP64 where kinetic energy exceeds half
+                           of E (FP32 elsewhere), FP64 dt reduction */
 
 typedef struct mh_ctx mh_ctx;
 
